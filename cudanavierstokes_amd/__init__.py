@@ -1,0 +1,12 @@
+"""MI355X-native explicit compressible Navier-Stokes time advance.
+
+The compute path is libcns.so (hand-written gfx950 HIP kernels behind the
+C ABI of include/cns.h); this package is its Python host binding.
+"""
+from ._lib import CnsConfig, CnsError, load  # noqa: F401
+from .solver import (Config, Solver, channel, clearSolver, copyField,  # noqa: F401
+                     initSolver, runSimulation, setDevice, setGPUParameters,
+                     taylor_green)
+
+__all__ = ["Config", "Solver", "CnsError", "taylor_green", "channel", "setDevice",
+           "setGPUParameters", "initSolver", "copyField", "runSimulation", "clearSolver", "load"]

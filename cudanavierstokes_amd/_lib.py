@@ -1,0 +1,96 @@
+"""ctypes binding of libcns.so (include/cns.h).
+
+The library is the product: if it is missing or fails to load, every entry
+point raises; there is no fallback implementation.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcns.so")
+
+CNS_OK = 0
+CNS_RK3_LOWSTORAGE, CNS_RK3_KUTTA, CNS_RK4 = 0, 1, 2
+CNS_FP64, CNS_FP32 = 0, 1
+
+
+class CnsConfig(C.Structure):
+    """Mirror of struct cns_config."""
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("stencil_size", C.c_int), ("stencil_visc", C.c_int),
+        ("periodic_x", C.c_int), ("nonuniform_x", C.c_int), ("forcing_z", C.c_int),
+        ("sponge", C.c_int), ("scheme", C.c_int), ("check_cfl", C.c_int),
+        ("check_bulk", C.c_int), ("fix_y_work", C.c_int), ("precision", C.c_int),
+        ("device", C.c_int),
+        ("len_x", C.c_double), ("len_y", C.c_double), ("len_z", C.c_double),
+        ("reynolds", C.c_double), ("prandtl", C.c_double), ("mach", C.c_double), ("gamma", C.c_double),
+        ("visc_exp", C.c_double), ("cfl", C.c_double), ("stretch", C.c_double),
+        ("twall_bot", C.c_double), ("twall_top", C.c_double), ("dpdz0", C.c_double),
+    ]
+
+
+# Python Config names (the reference's macro names) -> C struct fields
+C_NAMES = {"mx": "nx", "my": "ny", "mz": "nz", "forcing": "forcing_z", "Lx": "len_x", "Ly": "len_y",
+           "Lz": "len_z", "Re": "reynolds", "Pr": "prandtl", "Ma": "mach", "gam": "gamma",
+           "viscexp": "visc_exp"}
+
+
+EXPORTS = [
+    "cns_config_init", "cns_create", "cns_destroy", "cns_last_error", "cns_get_grid",
+    "cns_fill_taylor_green", "cns_fill_channel", "cns_set_state", "cns_get_state", "cns_rhs",
+    "cns_step", "cns_set_sponge", "cns_get_scalars", "cns_sync", "cns_profile_enable",
+    "cns_profile_read", "cns_time_steps", "cns_device_bytes",
+]
+
+_lib = None
+
+
+class CnsError(RuntimeError):
+    pass
+
+
+def load(path=None):
+    """Load libcns.so and declare the prototypes; raises if it is absent."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise CnsError(f"{p} is not built; run cudanavierstokes_amd/build.py (hipcc, gfx950)")
+    L = C.CDLL(p)
+    dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+    vp = C.c_void_p
+    L.cns_config_init.argtypes = [C.POINTER(CnsConfig)]
+    L.cns_create.argtypes = [C.POINTER(CnsConfig), C.POINTER(vp)]
+    L.cns_destroy.argtypes = [vp]
+    L.cns_last_error.restype = C.c_char_p
+    L.cns_last_error.argtypes = [vp]
+    L.cns_get_grid.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    L.cns_fill_taylor_green.argtypes = [vp, dp, dp, dp, dp, dp]
+    L.cns_fill_channel.argtypes = [vp, C.c_uint64, C.c_double, C.c_double, dp, dp, dp, dp, dp]
+    L.cns_set_state.argtypes = [vp, dp, dp, dp, dp, dp]
+    L.cns_get_state.argtypes = [vp, dp, dp, dp, dp, dp]
+    L.cns_rhs.argtypes = [vp, dp, dp, dp, dp, dp]
+    L.cns_step.argtypes = [vp, C.c_int, vp, vp, vp]
+    L.cns_set_sponge.argtypes = [vp, dp, dp, dp, dp, dp, dp, dp]
+    L.cns_get_scalars.argtypes = [vp, vp, vp, vp]
+    L.cns_sync.argtypes = [vp]
+    L.cns_profile_enable.argtypes = [vp, C.c_int]
+    L.cns_profile_read.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
+    L.cns_time_steps.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
+    L.cns_device_bytes.argtypes = [vp, C.POINTER(C.c_uint64)]
+    for name in EXPORTS:
+        if name != "cns_last_error":
+            getattr(L, name).restype = C.c_int
+    if path is None:
+        _lib = L
+    return L
+
+
+def check(rc, handle=None, what=""):
+    if rc != CNS_OK:
+        msg = load().cns_last_error(handle)
+        raise CnsError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")

@@ -1,0 +1,889 @@
+// cns_api.hip -- host side of the C ABI (include/cns.h): configuration,
+// grid and metric assembly, device buffers, the step loop and diagnostics.
+//
+// Reference counterparts:
+//   cns_create        setGPUParameters + initSolver (src/cuda_utils.cu:49-179,415-519)
+//   grid / metrics    initGrid, derivGrid (src/init.cpp:32-92,258-277)
+//   cns_set_state     copyField(0) (src/cuda_utils.cu:299-331)
+//   cns_get_state     copyField(1) (src/cuda_utils.cu:332-355)
+//   cns_rhs           calcRHS (src/cuda_main.cu:15-42)
+//   cns_step          runSimulation / runSimulationLowStorage (src/cuda_main.cu:44-186)
+//   dt, forcing       calcTimeStepPressGrad (src/cuda_main.cu:249-265), kept on the device
+//   cns_destroy       clearSolver (src/cuda_utils.cu:521)
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cns.h"
+#include "cns_aux.hpp"
+#include "cns_launch.hpp"
+
+using namespace cns;
+
+namespace {
+
+thread_local std::string g_err;  // errors without a handle
+
+// exact rational central-difference weights (order 2s), reference layout:
+// cF[it] multiplies f[i+it-s]-f[i+s-it]; cS[it] multiplies f[i+it-s]+f[i+s-it],
+// cS[s] is the centre weight (src/globals.h:69-95)
+int64_t igcd(int64_t a, int64_t b) {
+    if (a < 0) a = -a;
+    if (b < 0) b = -b;
+    while (b) {
+        int64_t t = a % b;
+        a = b;
+        b = t;
+    }
+    return a ? a : 1;
+}
+int64_t ifact(int n) {
+    int64_t f = 1;
+    for (int i = 2; i <= n; i++) f *= i;
+    return f;
+}
+void fd_weights(int s, double *cF, double *cS) {
+    const int64_t sf2 = ifact(s) * ifact(s);
+    int64_t cn = 0, cd = 1;
+    for (int k = 1; k <= s; k++) {
+        int64_t num = sf2, den = (int64_t)k * ifact(s - k) * ifact(s + k);
+        int64_t g = igcd(num, den);
+        num /= g;
+        den /= g;
+        const bool odd = (k % 2) == 1;
+        cF[s - k] = (odd ? -(double)num : (double)num) / (double)den;
+        int64_t n2 = 2 * num, d2 = den * k;
+        g = igcd(n2, d2);
+        n2 /= g;
+        d2 /= g;
+        cS[s - k] = (odd ? (double)n2 : -(double)n2) / (double)d2;
+        const int64_t sn = odd ? n2 : -n2;
+        int64_t nn = cn * d2 + sn * cd, dn = cd * d2;
+        g = igcd(nn, dn);
+        cn = nn / g;
+        cd = dn / g;
+    }
+    int64_t nc = -2 * cn, dc = cd;
+    const int64_t g = igcd(nc, dc);
+    nc /= g;
+    dc /= g;
+    cS[s] = (double)nc / (double)dc;
+}
+
+// SplitMix64 and its uniform double in [0,1): (x >> 11) * 2^-53
+struct SplitMix64 {
+    uint64_t x;
+    uint64_t next() {
+        uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double uniform() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
+};
+
+}  // namespace
+
+struct cns_solver {
+    cns_config cfg;
+    std::string err;
+    int S = 0, SV = 0;
+    long n = 0;
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    // grid (host)
+    std::vector<double> x, y, z, xp, xpp, dxv, vsx;
+    double dx = 0, d_dx = 0, d_dy = 0, d_dz = 0, d_d2x = 0, d_d2y = 0, d_d2z = 0, Rgas = 0, Ec = 0;
+    double cF[4] = {0}, cS[5] = {0}, cVF[4] = {0}, cVS[5] = {0};
+    // device
+    double *d_xp = nullptr, *d_vsx = nullptr, *d_dxv = nullptr, *d_spx = nullptr, *d_spz = nullptr;
+    double *d_ref[5] = {nullptr};
+    Scalars *d_sc = nullptr;
+    double *d_part = nullptr;
+    double *d_dil = nullptr;
+    int nsets = 0, nrhs = 0;
+    double *sets[4][5] = {{nullptr}};
+    double *rbuf[2][5] = {{nullptr}};
+    int cur = 0;
+    double *d_par1 = nullptr, *d_par2 = nullptr, *d_time = nullptr;
+    int par_cap = 0;
+    bool visc_from_state = true;
+    bool have_state = false;
+    // launch geometry
+    int ntx = 0, nty = 0, nzc = 0, zc = 0, nblocks = 0;
+    Params base{};
+    // profiling
+    bool sync_check = false;  // CNS_SYNC_CHECK=1: synchronise and check after every launch
+    bool prof = false;
+    std::vector<hipEvent_t> ev;
+    int prof_n = 0;
+    uint64_t dev_bytes = 0;
+};
+
+#define HIPCHK(s, expr)                                                                                   \
+    do {                                                                                                  \
+        hipError_t e_ = (expr);                                                                           \
+        if (e_ != hipSuccess) {                                                                           \
+            (s)->err = std::string(#expr) + ": " + hipGetErrorString(e_);                                 \
+            return CNS_ERR_HIP;                                                                           \
+        }                                                                                                 \
+    } while (0)
+
+// After a launch: report launch errors; with sync_check also wait for the
+// kernel and report its execution status, naming the kernel.
+static int launched(cns_solver *s, const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && s->sync_check) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) {
+        s->err = std::string(what) + ": " + hipGetErrorString(e);
+        return CNS_ERR_HIP;
+    }
+    return CNS_OK;
+}
+#define LAUNCHED(s, what)                   \
+    do {                                    \
+        int rc_ = launched((s), (what));    \
+        if (rc_) return rc_;                \
+    } while (0)
+
+static int fail(cns_solver *s, int code, const std::string &msg) {
+    if (s)
+        s->err = msg;
+    else
+        g_err = msg;
+    return code;
+}
+
+static int dalloc(cns_solver *s, double **p, size_t count) {
+    HIPCHK(s, hipMalloc((void **)p, count * sizeof(double)));
+    s->dev_bytes += count * sizeof(double);
+    return CNS_OK;
+}
+
+// grid and metrics: src/init.cpp:32-92 (initGrid), 258-277 (derivGrid),
+// src/cuda_utils.cu:60-139 (inverse spacings, cell widths, stretched
+// second-derivative weights)
+static void build_grid(cns_solver *s) {
+    const cns_config &c = s->cfg;
+    const int mx = c.nx, my = c.ny, mz = c.nz, S = s->S, SV = s->SV;
+    s->x.assign(mx, 0.0);
+    s->xp.assign(mx, 0.0);
+    s->xpp.assign(mx, 0.0);
+    s->dxv.assign(mx, 0.0);
+    s->y.assign(my, 0.0);
+    s->z.assign(mz, 0.0);
+    s->dx = c.len_x * (1.0) / (mx);
+    std::vector<double> xn(mx + 1);
+    for (int i = 0; i < mx + 1; i++) xn[i] = tanh(c.stretch * ((i * 1.0) / mx - 0.5)) / tanh(c.stretch * 0.5);
+    for (int i = 0; i < mx; i++) s->x[i] = c.len_x * (1.0 + (xn[i] + xn[i + 1]) / 2.0) / 2.0;
+    // derivatives of x(xi) with odd/2Lx-odd mirrored ghosts, advective order
+    std::vector<double> fb(mx + 2 * S);
+    for (int i = S; i < mx + S; i++) fb[i] = s->x[i - S];
+    for (int i = 0; i < S; i++) {
+        fb[i] = -fb[2 * S - i - 1];
+        fb[mx + S + i] = 2 * c.len_x - fb[mx + S - i - 1];
+    }
+    for (int i = 0; i < mx; i++) {
+        double df = 0.0, d2f = s->cS[S] * fb[i + S] / s->dx / s->dx;
+        for (int it = 0; it < S; it++) {
+            df += s->cF[it] * (fb[i + it] - fb[i + S * 2 - it]) / s->dx;
+            d2f += s->cS[it] * (fb[i + it] + fb[i + S * 2 - it]) / s->dx / s->dx;
+        }
+        s->xp[i] = 1.0 / df;
+        s->xpp[i] = d2f;
+    }
+    if (!c.nonuniform_x) {
+        for (int i = 0; i < mx; i++) s->x[i] = c.len_x * (0.5 + i * 1.0) / (mx);
+        s->dx = s->x[1] - s->x[0];
+    }
+    for (int j = 0; j < my; j++) s->y[j] = c.len_y * (0.5 + j * 1.0) / (my);
+    for (int k = 0; k < mz; k++) s->z[k] = c.len_z * (0.5 + k * 1.0) / (mz);
+    s->d_dx = 1.0 / (s->dx);
+    s->d_dy = 1.0 / (s->y[1] - s->y[0]);
+    s->d_dz = 1.0 / (s->z[1] - s->z[0]);
+    s->d_d2x = s->d_dx * s->d_dx;
+    s->d_d2y = s->d_dy * s->d_dy;
+    s->d_d2z = s->d_dz * s->d_dz;
+    s->dxv[0] = (s->x[1] + s->x[0]) / 2.0;
+    for (int i = 1; i < mx - 1; i++) s->dxv[i] = (s->x[i + 1] - s->x[i - 1]) / 2.0;
+    s->dxv[mx - 1] = c.len_x - (s->x[mx - 1] + s->x[mx - 2]) / 2.0;
+    // f_xx = xi_x^2 f_xixi + xi_xx f_xi folded into (2SV+1) x mx weights
+    s->vsx.assign((size_t)mx * (2 * SV + 1), 0.0);
+    const double *xp = s->xp.data(), *xpp = s->xpp.data();
+    for (int it = 0; it < SV; it++)
+        for (int i = 0; i < mx; i++)
+            s->vsx[i + it * mx] = (s->cVS[it] * (xp[i] * xp[i]) * s->d_d2x -
+                                   s->cVF[it] * xpp[i] * (xp[i] * xp[i] * xp[i]) * s->d_dx);
+    for (int i = 0; i < mx; i++) s->vsx[i + SV * mx] = s->cVS[SV] * (xp[i] * xp[i]) * s->d_d2x;
+    for (int it = SV + 1; it < 2 * SV + 1; it++)
+        for (int i = 0; i < mx; i++)
+            s->vsx[i + it * mx] = (s->cVS[2 * SV - it] * (xp[i] * xp[i]) * s->d_d2x +
+                                   s->cVF[2 * SV - it] * xpp[i] * (xp[i] * xp[i] * xp[i]) * s->d_dx);
+}
+
+static void launch_geometry(cns_solver *s) {
+    const cns_config &c = s->cfg;
+    s->ntx = (c.nx + TILE_X - 1) / TILE_X;
+    s->nty = (c.ny + TILE_Y - 1) / TILE_Y;
+    const int cols = s->ntx * s->nty;
+    int want = (1024 + cols - 1) / cols;  // aim at ~1024 workgroups
+    int maxc = c.nz / (2 * s->S);         // keep chunks longer than the window
+    if (maxc < 1) maxc = 1;
+    if (want > maxc) want = maxc;
+    if (want < 1) want = 1;
+    s->zc = (c.nz + want - 1) / want;
+    s->nzc = (c.nz + s->zc - 1) / s->zc;
+    s->nblocks = s->ntx * s->nty * s->nzc;
+}
+
+static void fill_params(cns_solver *s) {
+    const cns_config &c = s->cfg;
+    Params &P = s->base;
+    std::memset(&P, 0, sizeof(P));
+    P.mx = c.nx;
+    P.my = c.ny;
+    P.mz = c.nz;
+    P.zc = s->zc;
+    P.ntx = s->ntx;
+    P.nty = s->nty;
+    P.nzc = s->nzc;
+    P.forcing = c.forcing_z;
+    P.sponge = c.sponge;
+    P.fix_y_work = c.fix_y_work;
+    P.unit_visc = (c.visc_exp == 1.0);
+    P.xcd_remap = 1;
+    P.d_dx = s->d_dx;
+    P.d_dy = s->d_dy;
+    P.d_dz = s->d_dz;
+    P.d_d2x = s->d_d2x;
+    P.d_d2y = s->d_d2y;
+    P.d_d2z = s->d_d2z;
+    P.Rgas = s->Rgas;
+    P.gam = c.gamma;
+    P.cvInv = (c.gamma - 1.0) / s->Rgas;
+    P.Re = c.reynolds;
+    P.Pr = c.prandtl;
+    P.Ec = s->Ec;
+    P.viscexp = c.visc_exp;
+    P.invPrEc = 1.0 / (c.prandtl * s->Ec);
+    P.twall_bot = c.twall_bot;
+    P.twall_top = c.twall_top;
+    for (int i = 0; i < 4; i++) {
+        P.cF[i] = s->cF[i];
+        P.cVF[i] = s->cVF[i];
+    }
+    for (int i = 0; i < 5; i++) {
+        P.cS[i] = s->cS[i];
+        P.cVS[i] = s->cVS[i];
+    }
+    P.xp = s->d_xp;
+    P.vsx = s->d_vsx;
+    P.dxv = s->d_dxv;
+    P.spx = s->d_spx;
+    P.spz = s->d_spz;
+    for (int q = 0; q < 5; q++) P.ref[q] = s->d_ref[q];
+    P.dt = &s->d_sc->dt;
+    P.dpdz = &s->d_sc->dpdz;
+    P.visc_max_out = &s->d_sc->visc_max;
+}
+
+extern "C" int cns_config_init(cns_config *c) {
+    if (!c) return fail(nullptr, CNS_ERR_ARG, "cns_config_init: null config");
+    std::memset(c, 0, sizeof(*c));
+    c->nx = c->ny = c->nz = 64;
+    c->stencil_size = 2;
+    c->stencil_visc = 2;
+    c->periodic_x = 1;
+    c->nonuniform_x = 0;
+    c->forcing_z = 0;
+    c->sponge = 0;
+    c->scheme = CNS_RK3_LOWSTORAGE;
+    c->check_cfl = 10;
+    c->check_bulk = 10;
+    c->fix_y_work = 0;
+    c->precision = CNS_FP64;
+    c->device = 0;
+    c->len_x = c->len_y = c->len_z = 2.0 * M_PI;
+    c->reynolds = 1600.0;
+    c->prandtl = 0.71;
+    c->mach = 0.1;
+    c->gamma = 1.4;
+    c->visc_exp = 1.0;
+    c->cfl = 0.5;
+    c->stretch = 1.8;
+    c->twall_bot = 1.0;
+    c->twall_top = 1.0;
+    c->dpdz0 = 0.00372;
+    return CNS_OK;
+}
+
+extern "C" const char *cns_last_error(const cns_solver *s) { return s ? s->err.c_str() : g_err.c_str(); }
+
+static int validate(const cns_config *c, std::string &msg) {
+    if (c->stencil_size < 1 || c->stencil_size > 4) {
+        msg = "stencil_size must be 1..4 (orders 2, 4, 6, 8)";
+        return CNS_ERR_ARG;
+    }
+    if (c->stencil_visc < 1 || c->stencil_visc > c->stencil_size) {
+        msg = "stencil_visc must be 1..stencil_size";
+        return CNS_ERR_ARG;
+    }
+    const int need = 2 * c->stencil_size;
+    if (c->nx < need || c->ny < need || c->nz < need) {
+        msg = "every grid dimension must be at least 2*stencil_size";
+        return CNS_ERR_ARG;
+    }
+    if ((long)c->nx * c->ny * c->nz > (1L << 31) - 1) {
+        msg = "grid larger than 2^31-1 points";
+        return CNS_ERR_ARG;
+    }
+    if (c->scheme < 0 || c->scheme > 2) {
+        msg = "scheme must be CNS_RK3_LOWSTORAGE, CNS_RK3_KUTTA or CNS_RK4";
+        return CNS_ERR_ARG;
+    }
+    if (c->check_cfl < 1 || c->check_bulk < 1) {
+        msg = "check_cfl and check_bulk must be >= 1";
+        return CNS_ERR_ARG;
+    }
+    if (!(c->len_x > 0 && c->len_y > 0 && c->len_z > 0 && c->reynolds > 0 && c->prandtl > 0 && c->mach > 0 && c->gamma > 1 &&
+          c->cfl > 0)) {
+        msg = "lengths, Re, Pr, Ma, cfl must be positive and gam > 1";
+        return CNS_ERR_ARG;
+    }
+    if (c->precision != CNS_FP64) {
+        msg = "precision: only CNS_FP64 is available in this build";
+        return CNS_ERR_UNSUPPORTED;
+    }
+    return CNS_OK;
+}
+
+extern "C" int cns_destroy(cns_solver *s) {
+    if (!s) return CNS_OK;
+    if (s->dev >= 0) (void)hipSetDevice(s->dev);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    auto fr = [](double *p) {
+        if (p) (void)hipFree(p);
+    };
+    fr(s->d_xp);
+    fr(s->d_vsx);
+    fr(s->d_dxv);
+    fr(s->d_spx);
+    fr(s->d_spz);
+    for (int q = 0; q < 5; q++) fr(s->d_ref[q]);
+    if (s->d_sc) (void)hipFree(s->d_sc);
+    fr(s->d_part);
+    fr(s->d_dil);
+    for (int a = 0; a < 4; a++)
+        for (int q = 0; q < 5; q++) fr(s->sets[a][q]);
+    for (int a = 0; a < 2; a++)
+        for (int q = 0; q < 5; q++) fr(s->rbuf[a][q]);
+    fr(s->d_par1);
+    fr(s->d_par2);
+    fr(s->d_time);
+    for (auto &e : s->ev) (void)hipEventDestroy(e);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+    return CNS_OK;
+}
+
+extern "C" int cns_create(const cns_config *cfg, cns_solver **out) {
+    if (!cfg || !out) return fail(nullptr, CNS_ERR_ARG, "cns_create: null argument");
+    *out = nullptr;
+    std::string msg;
+    int rc = validate(cfg, msg);
+    if (rc) return fail(nullptr, rc, msg);
+    int ndev = 0;
+    const hipError_t de = hipGetDeviceCount(&ndev);
+    if (de != hipSuccess || ndev <= 0)
+        return fail(nullptr, CNS_ERR_HIP, std::string("cns_create: no HIP device available (hipGetDeviceCount: ") +
+                                              hipGetErrorString(de) + ", count " + std::to_string(ndev) + ")");
+    if (cfg->device < 0 || cfg->device >= ndev) return fail(nullptr, CNS_ERR_ARG, "cns_create: bad device ordinal");
+    cns_solver *s = new cns_solver();
+    s->cfg = *cfg;
+    s->dev = cfg->device;
+    {
+        const char *sc = std::getenv("CNS_SYNC_CHECK");
+        s->sync_check = sc && sc[0] == '1';
+    }
+    s->S = cfg->stencil_size;
+    s->SV = cfg->stencil_visc;
+    s->n = (long)cfg->nx * cfg->ny * cfg->nz;
+    fd_weights(s->S, s->cF, s->cS);
+    fd_weights(s->SV, s->cVF, s->cVS);
+    s->Ec = ((cfg->gamma - 1.0) * cfg->mach * cfg->mach);
+    s->Rgas = (1.0 / (cfg->gamma * cfg->mach * cfg->mach));
+    build_grid(s);
+    launch_geometry(s);
+    auto bail = [&](int code) {
+        g_err = s->err;
+        cns_destroy(s);
+        return code;
+    };
+    if (hipSetDevice(s->dev) != hipSuccess) {
+        s->err = "hipSetDevice failed";
+        return bail(CNS_ERR_HIP);
+    }
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+        s->err = "hipStreamCreate failed";
+        return bail(CNS_ERR_HIP);
+    }
+    const int mx = cfg->nx, mz = cfg->nz;
+    const long n = s->n;
+    s->nsets = (cfg->scheme == CNS_RK3_LOWSTORAGE) ? 2 : (cfg->scheme == CNS_RK3_KUTTA ? 3 : 4);
+    s->nrhs = (cfg->scheme == CNS_RK4) ? 1 : 2;
+    if ((rc = dalloc(s, &s->d_xp, mx)) || (rc = dalloc(s, &s->d_vsx, s->vsx.size())) ||
+        (rc = dalloc(s, &s->d_dxv, mx)) || (rc = dalloc(s, &s->d_spx, mx)) || (rc = dalloc(s, &s->d_spz, mz)) ||
+        (rc = dalloc(s, &s->d_part, (size_t)NPART * RED_BLOCKS)) || (rc = dalloc(s, &s->d_dil, n)))
+        return bail(rc);
+    for (int q = 0; q < 5; q++)
+        if ((rc = dalloc(s, &s->d_ref[q], (size_t)mx * mz))) return bail(rc);
+    for (int a = 0; a < s->nsets; a++)
+        for (int q = 0; q < 5; q++)
+            if ((rc = dalloc(s, &s->sets[a][q], n))) return bail(rc);
+    for (int a = 0; a < s->nrhs; a++)
+        for (int q = 0; q < 5; q++)
+            if ((rc = dalloc(s, &s->rbuf[a][q], n))) return bail(rc);
+    if (hipMalloc((void **)&s->d_sc, sizeof(Scalars)) != hipSuccess) {
+        s->err = "hipMalloc(scalars) failed";
+        return bail(CNS_ERR_NOMEM);
+    }
+    Scalars h{};
+    h.dt = 0.0;
+    h.dpdz = cfg->forcing_z ? cfg->dpdz0 : 0.0;
+    h.time = 0.0;
+    if (hipMemcpy(s->d_sc, &h, sizeof(h), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->d_xp, s->xp.data(), mx * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->d_vsx, s->vsx.data(), s->vsx.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->d_dxv, s->dxv.data(), mx * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+        s->err = "initial host-to-device copies failed";
+        return bail(CNS_ERR_HIP);
+    }
+    std::vector<double> zeros((size_t)mx * mz > (size_t)mz ? (size_t)mx * mz : (size_t)mz, 0.0);
+    if (hipMemcpy(s->d_spx, zeros.data(), mx * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->d_spz, zeros.data(), mz * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+        s->err = "sponge initialisation failed";
+        return bail(CNS_ERR_HIP);
+    }
+    for (int q = 0; q < 5; q++)
+        if (hipMemcpy(s->d_ref[q], zeros.data(), (size_t)mx * mz * sizeof(double), hipMemcpyHostToDevice) !=
+            hipSuccess) {
+            s->err = "sponge initialisation failed";
+            return bail(CNS_ERR_HIP);
+        }
+    fill_params(s);
+    *out = s;
+    return CNS_OK;
+}
+
+extern "C" int cns_get_grid(const cns_solver *s, double *x, double *y, double *z, double *xp, double *xpp,
+                            double *dxv) {
+    if (!s) return fail(nullptr, CNS_ERR_ARG, "null solver");
+    const int mx = s->cfg.nx;
+    if (x) std::memcpy(x, s->x.data(), mx * sizeof(double));
+    if (xp) std::memcpy(xp, s->xp.data(), mx * sizeof(double));
+    if (xpp) std::memcpy(xpp, s->xpp.data(), mx * sizeof(double));
+    if (dxv) std::memcpy(dxv, s->dxv.data(), mx * sizeof(double));
+    if (y) std::memcpy(y, s->y.data(), s->cfg.ny * sizeof(double));
+    if (z) std::memcpy(z, s->z.data(), s->cfg.nz * sizeof(double));
+    return CNS_OK;
+}
+
+// src/init.cpp:126-148 (initCHIT): V0 = T0 = R0 = 1, P0 = T0*Rgas
+extern "C" int cns_fill_taylor_green(const cns_solver *s, double *r, double *u, double *v, double *w, double *e) {
+    if (!s || !r || !u || !v || !w || !e) return fail(nullptr, CNS_ERR_ARG, "cns_fill_taylor_green: null argument");
+    const cns_config &c = s->cfg;
+    const double V0 = 1.0, T0 = 1.0, P0 = T0 * s->Rgas, R0 = 1.0;
+    const int mx = c.nx, my = c.ny, mz = c.nz;
+    for (int k = 0; k < mz; k++) {
+        const double fz = 2 * M_PI * s->z[k] / c.len_z;
+        for (int j = 0; j < my; j++) {
+            const double fy = 2 * M_PI * s->y[j] / c.len_y;
+            for (int i = 0; i < mx; i++) {
+                const double fx = 2 * M_PI * s->x[i] / c.len_x;
+                const long g = i + (long)mx * (j + (long)my * k);
+                u[g] = V0 * sin(fx / 1.0) * cos(fy / 1.0) * cos(fz / 1.0);
+                v[g] = -V0 * cos(fx / 1.0) * sin(fy / 1.0) * cos(fz / 1.0);
+                w[g] = 0.0;
+                const double press =
+                    P0 + 1.0 / 16.0 * R0 * V0 * V0 * (cos(2.0 * fx / 1.0) + cos(2.0 * fy / 1.0)) * (cos(2.0 * fz / 1.0) + 2.0);
+                r[g] = press / s->Rgas / T0;
+                e[g] = press / (c.gamma - 1.0) + 0.5 * r[g] * (pow(u[g], 2) + pow(v[g], 2) + pow(w[g], 2));
+            }
+        }
+    }
+    return CNS_OK;
+}
+
+// Isothermal channel: rho = 1, T = 1, p = Rgas; streamwise (z) velocity
+// 1.5*ubulk*(1-(x/h-1)^2) with h = Lx/2; every component perturbed by
+// amp*ubulk*(2U-1), U from SplitMix64(seed) drawn in storage order, three
+// draws (u, v, w) per point.
+extern "C" int cns_fill_channel(const cns_solver *s, uint64_t seed, double ubulk, double amp, double *r, double *u,
+                                double *v, double *w, double *e) {
+    if (!s || !r || !u || !v || !w || !e) return fail(nullptr, CNS_ERR_ARG, "cns_fill_channel: null argument");
+    const cns_config &c = s->cfg;
+    const double T0 = 1.0, P0 = T0 * s->Rgas, R0 = 1.0, hh = c.len_x / 2.0;
+    SplitMix64 rng{seed};
+    const int mx = c.nx, my = c.ny, mz = c.nz;
+    for (long g = 0; g < (long)mx * my * mz; g++) {
+        const int i = (int)(g % mx);
+        const double xi = s->x[i] / hh - 1.0;
+        const double wm = 1.5 * ubulk * (1.0 - xi * xi);
+        const double a1 = rng.uniform(), a2 = rng.uniform(), a3 = rng.uniform();
+        u[g] = amp * ubulk * (2.0 * a1 - 1.0);
+        v[g] = amp * ubulk * (2.0 * a2 - 1.0);
+        w[g] = wm + amp * ubulk * (2.0 * a3 - 1.0);
+        r[g] = R0;
+        e[g] = P0 / (c.gamma - 1.0) + 0.5 * r[g] * (u[g] * u[g] + v[g] * v[g] + w[g] * w[g]);
+    }
+    return CNS_OK;
+}
+
+// Pointers a stage does not use still point at valid device memory (the
+// stage input), so no kernel argument is ever a null device address.
+static void fill_unused(StagePtrs<double> &A) {
+    for (int q = 0; q < 5; q++) {
+        double *in = const_cast<double *>(A.qin[q]);
+        if (!A.qout[q]) A.qout[q] = in;
+        if (!A.rout[q]) A.rout[q] = in;
+        if (!A.acc[q]) A.acc[q] = in;
+        if (!A.ra[q]) A.ra[q] = in;
+        if (!A.rb[q]) A.rb[q] = in;
+        if (!A.q0[q]) A.q0[q] = in;
+    }
+}
+
+static StagePtrs<double> ptrs_in(cns_solver *s, int set) {
+    StagePtrs<double> A;
+    std::memset(&A, 0, sizeof(A));
+    for (int q = 0; q < 5; q++) A.qin[q] = s->sets[set][q];
+    A.dil = s->d_dil;
+    fill_unused(A);
+    return A;
+}
+
+extern "C" int cns_set_state(cns_solver *s, const double *r, const double *u, const double *v, const double *w,
+                             const double *e) {
+    if (!s || !r || !u || !v || !w || !e) return fail(s, CNS_ERR_ARG, "cns_set_state: null argument");
+    HIPCHK(s, hipSetDevice(s->dev));
+    const size_t b = s->n * sizeof(double);
+    const double *src[5] = {r, u, v, w, e};
+    s->cur = 0;
+    for (int q = 0; q < 5; q++) HIPCHK(s, hipMemcpyAsync(s->sets[0][q], src[q], b, hipMemcpyHostToDevice, s->stream));
+    hipLaunchKernelGGL(k_to_conserved<double>, dim3(1024), dim3(256), 0, s->stream, s->n, s->sets[0][0],
+                       s->sets[0][1], s->sets[0][2], s->sets[0][3]);
+    LAUNCHED(s, "k_to_conserved<double>");
+    // time restarts at zero (initDevice, src/cuda_utils.cu:386)
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(1), 0, s->stream, &s->d_sc->time, 1L, 0.0);
+    LAUNCHED(s, "k_fill");
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    s->visc_from_state = true;
+    s->have_state = true;
+    return CNS_OK;
+}
+
+extern "C" int cns_get_state(cns_solver *s, double *r, double *u, double *v, double *w, double *e) {
+    if (!s || !r || !u || !v || !w || !e) return fail(s, CNS_ERR_ARG, "cns_get_state: null argument");
+    if (!s->have_state) return fail(s, CNS_ERR_STATE, "cns_get_state: no state set");
+    HIPCHK(s, hipSetDevice(s->dev));
+    const size_t b = s->n * sizeof(double);
+    double *const *q = s->sets[s->cur];
+    // velocities into the dilatation buffer and the first rhs buffer
+    double *tu = s->rbuf[0][0], *tv = s->rbuf[0][1], *tw = s->rbuf[0][2];
+    hipLaunchKernelGGL(k_to_velocity<double>, dim3(1024), dim3(256), 0, s->stream, s->n, q[0], q[1], q[2], q[3], tu,
+                       tv, tw);
+    LAUNCHED(s, "k_to_velocity<double>");
+    HIPCHK(s, hipMemcpyAsync(r, q[0], b, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(s, hipMemcpyAsync(u, tu, b, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(s, hipMemcpyAsync(v, tv, b, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(s, hipMemcpyAsync(w, tw, b, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(s, hipMemcpyAsync(e, q[4], b, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    return CNS_OK;
+}
+
+// dilatation + fused RHS/update of one stage
+static int run_stage(cns_solver *s, int in_set, Params P, StagePtrs<double> A) {
+    const bool wallx = !s->cfg.periodic_x, nonuni = s->cfg.nonuniform_x != 0;
+    for (int q = 0; q < 5; q++) A.qin[q] = s->sets[in_set][q];
+    A.dil = s->d_dil;
+    fill_unused(A);
+    HIPCHK(s, launch_dil<double>(s->SV, wallx, nonuni, s->nblocks, s->stream, P, A, s->d_dil));
+    LAUNCHED(s, "k_dil");
+    if (P.want_visc_max) {
+        hipLaunchKernelGGL(k_reset_visc, dim3(1), dim3(1), 0, s->stream, s->d_sc);
+        LAUNCHED(s, "k_reset_visc");
+    }
+    if (s->prof) {
+        const size_t need = 2 * (size_t)(s->prof_n + 1);
+        while (s->ev.size() < need) {
+            hipEvent_t e;
+            HIPCHK(s, hipEventCreate(&e));
+            s->ev.push_back(e);
+        }
+        HIPCHK(s, hipEventRecord(s->ev[2 * s->prof_n], s->stream));
+    }
+    HIPCHK(s, launch_rhs<double>(s->S, s->SV, wallx, nonuni, s->nblocks, s->stream, P, A));
+    LAUNCHED(s, "k_rhs");
+    if (s->prof) {
+        HIPCHK(s, hipEventRecord(s->ev[2 * s->prof_n + 1], s->stream));
+        s->prof_n++;
+    }
+    return CNS_OK;
+}
+
+extern "C" int cns_rhs(cns_solver *s, double *rr, double *ru, double *rv, double *rw, double *re) {
+    if (!s || !rr || !ru || !rv || !rw || !re) return fail(s, CNS_ERR_ARG, "cns_rhs: null argument");
+    if (!s->have_state) return fail(s, CNS_ERR_STATE, "cns_rhs: no state set");
+    HIPCHK(s, hipSetDevice(s->dev));
+    Params P = s->base;
+    P.mode = MODE_RHS;
+    P.want_visc_max = 1;
+    StagePtrs<double> A;
+    std::memset(&A, 0, sizeof(A));
+    for (int q = 0; q < 5; q++) A.rout[q] = s->rbuf[0][q];
+    int rc = run_stage(s, s->cur, P, A);
+    if (rc) return rc;
+    s->visc_from_state = false;
+    double *dst[5] = {rr, ru, rv, rw, re};
+    const size_t b = s->n * sizeof(double);
+    for (int q = 0; q < 5; q++) HIPCHK(s, hipMemcpyAsync(dst[q], s->rbuf[0][q], b, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    return CNS_OK;
+}
+
+static const double SMR_ALPHA[3] = {0., -17. / 60., -5. / 12.};
+static const double SMR_BETA[3] = {8. / 15., 5. / 12., 3. / 4.};
+
+// queue nsteps steps on the stream; outputs stay on the device
+static int queue_steps(cns_solver *s, int nsteps, bool want_par, bool want_time) {
+    const cns_config &c = s->cfg;
+    const long n = s->n;
+    (void)n;
+    for (int istep = 0; istep < nsteps; istep++) {
+        StagePtrs<double> cur = ptrs_in(s, s->cur);
+        const bool cfl_step = (istep % c.check_cfl) == 0;
+        const bool bulk_step = want_par && (istep % c.check_bulk) == 0;
+        if (cfl_step) {
+            const int vfs = s->visc_from_state ? 1 : 0;
+            if (vfs) {
+                hipLaunchKernelGGL(k_reset_visc, dim3(1), dim3(1), 0, s->stream, s->d_sc);
+                LAUNCHED(s, "k_reset_visc");
+            }
+            hipLaunchKernelGGL(k_cfl<double>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s->stream, s->base, cur, vfs,
+                               s->d_sc);
+            LAUNCHED(s, "k_cfl<double>");
+            hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(1), 0, s->stream, c.cfl, s->d_sc);
+            LAUNCHED(s, "k_dt_finalize");
+            s->visc_from_state = false;
+        }
+        const int what = ((cfl_step && c.forcing_z) ? 1 : 0) | (bulk_step ? 2 : 0);
+        if (what) {
+            hipLaunchKernelGGL(k_partials<double>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s->stream, s->base, cur,
+                               s->d_part);
+            LAUNCHED(s, "k_partials<double>");
+        }
+        if (what & 1) {
+            hipLaunchKernelGGL(k_final, dim3(1), dim3(RED_THREADS), 0, s->stream, s->d_part, s->d_sc, 1, c.forcing_z,
+                               c.len_x * c.len_y * c.len_z, (double *)nullptr, (double *)nullptr, istep);
+            LAUNCHED(s, "k_final");
+        }
+        hipLaunchKernelGGL(k_advance_time, dim3(1), dim3(1), 0, s->stream, s->d_sc, want_time ? s->d_time : nullptr,
+                           istep);
+        LAUNCHED(s, "k_advance_time");
+        if (what & 2) {
+            hipLaunchKernelGGL(k_final, dim3(1), dim3(RED_THREADS), 0, s->stream, s->d_part, s->d_sc, 2, c.forcing_z,
+                               c.len_x * c.len_y * c.len_z, s->d_par1, s->d_par2, istep);
+            LAUNCHED(s, "k_final");
+        }
+        const bool need_visc = ((istep + 1) % c.check_cfl == 0) || (istep == nsteps - 1);
+        Params P = s->base;
+        int rc;
+        if (c.scheme == CNS_RK3_LOWSTORAGE) {
+            const int a = s->cur, b = 1 - s->cur;
+            int in[3] = {a, b, a}, outs[3] = {b, a, b};
+            for (int st = 0; st < 3; st++) {
+                P.mode = MODE_SMR;
+                P.stage = st;
+                P.alpha = SMR_ALPHA[st];
+                P.beta = SMR_BETA[st];
+                P.want_visc_max = (st == 2 && need_visc) ? 1 : 0;
+                StagePtrs<double> A;
+                std::memset(&A, 0, sizeof(A));
+                for (int q = 0; q < 5; q++) {
+                    A.qout[q] = s->sets[outs[st]][q];
+                    if (st == 0) A.rout[q] = s->rbuf[0][q];
+                    if (st == 1) {
+                        A.ra[q] = s->rbuf[0][q];
+                        A.rout[q] = s->rbuf[1][q];
+                    }
+                    if (st == 2) A.ra[q] = s->rbuf[1][q];
+                }
+                if ((rc = run_stage(s, in[st], P, A))) return rc;
+            }
+            s->cur = b;
+        } else if (c.scheme == CNS_RK3_KUTTA) {
+            const int q0 = s->cur, s1 = (s->cur + 1) % 3, s2 = (s->cur + 2) % 3;
+            int in[3] = {q0, s1, s2}, outs[3] = {s1, s2, q0};
+            for (int st = 0; st < 3; st++) {
+                P.mode = MODE_KUTTA;
+                P.stage = st;
+                P.want_visc_max = (st == 2 && need_visc) ? 1 : 0;
+                StagePtrs<double> A;
+                std::memset(&A, 0, sizeof(A));
+                for (int q = 0; q < 5; q++) {
+                    A.qout[q] = s->sets[outs[st]][q];
+                    A.q0[q] = s->sets[q0][q];
+                    if (st < 2) A.rout[q] = s->rbuf[st][q];
+                    A.ra[q] = s->rbuf[0][q];
+                    A.rb[q] = s->rbuf[1][q];
+                }
+                if ((rc = run_stage(s, in[st], P, A))) return rc;
+            }
+            s->cur = q0;
+        } else {
+            // sets: A = current, B/D stage inputs, C accumulator
+            const int sa = s->cur, sb = (s->cur + 1) % 4, scc = (s->cur + 2) % 4, sd = (s->cur + 3) % 4;
+            int in[4] = {sa, sb, sd, sb}, outs[4] = {sb, sd, sb, sa};
+            for (int st = 0; st < 4; st++) {
+                P.mode = MODE_RK4;
+                P.stage = st;
+                P.want_visc_max = (st == 3 && need_visc) ? 1 : 0;
+                StagePtrs<double> A;
+                std::memset(&A, 0, sizeof(A));
+                for (int q = 0; q < 5; q++) {
+                    A.qout[q] = s->sets[outs[st]][q];
+                    A.q0[q] = s->sets[sa][q];
+                    A.acc[q] = s->sets[scc][q];
+                }
+                if ((rc = run_stage(s, in[st], P, A))) return rc;
+            }
+            s->cur = sa;
+        }
+    }
+    return CNS_OK;
+}
+
+static int ensure_par(cns_solver *s, int nsteps) {
+    if (nsteps <= s->par_cap) return CNS_OK;
+    if (s->d_par1) (void)hipFree(s->d_par1);
+    if (s->d_par2) (void)hipFree(s->d_par2);
+    if (s->d_time) (void)hipFree(s->d_time);
+    s->d_par1 = s->d_par2 = s->d_time = nullptr;
+    HIPCHK(s, hipMalloc((void **)&s->d_par1, nsteps * sizeof(double)));
+    HIPCHK(s, hipMalloc((void **)&s->d_par2, nsteps * sizeof(double)));
+    HIPCHK(s, hipMalloc((void **)&s->d_time, nsteps * sizeof(double)));
+    s->par_cap = nsteps;
+    return CNS_OK;
+}
+
+extern "C" int cns_step(cns_solver *s, int nsteps, double *kin, double *enst, double *time) {
+    if (!s) return fail(nullptr, CNS_ERR_ARG, "null solver");
+    if (nsteps < 0) return fail(s, CNS_ERR_ARG, "cns_step: nsteps < 0");
+    if (!s->have_state) return fail(s, CNS_ERR_STATE, "cns_step: no state set");
+    if (nsteps == 0) return CNS_OK;
+    HIPCHK(s, hipSetDevice(s->dev));
+    const bool want_par = kin || enst;
+    int rc;
+    if ((rc = ensure_par(s, nsteps))) return rc;
+    if (want_par) {
+        const double nan = std::nan("");
+        hipLaunchKernelGGL(k_fill, dim3(64), dim3(256), 0, s->stream, s->d_par1, (long)nsteps, nan);
+        hipLaunchKernelGGL(k_fill, dim3(64), dim3(256), 0, s->stream, s->d_par2, (long)nsteps, nan);
+        LAUNCHED(s, "k_fill");
+    }
+    if ((rc = queue_steps(s, nsteps, want_par, time != nullptr))) return rc;
+    const size_t b = nsteps * sizeof(double);
+    if (kin) HIPCHK(s, hipMemcpyAsync(kin, s->d_par1, b, hipMemcpyDeviceToHost, s->stream));
+    if (enst) HIPCHK(s, hipMemcpyAsync(enst, s->d_par2, b, hipMemcpyDeviceToHost, s->stream));
+    if (time) HIPCHK(s, hipMemcpyAsync(time, s->d_time, b, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    return CNS_OK;
+}
+
+extern "C" int cns_time_steps(cns_solver *s, int nsteps, double *ms) {
+    if (!s || !ms) return fail(s, CNS_ERR_ARG, "cns_time_steps: null argument");
+    if (!s->have_state) return fail(s, CNS_ERR_STATE, "cns_time_steps: no state set");
+    HIPCHK(s, hipSetDevice(s->dev));
+    hipEvent_t a, b;
+    HIPCHK(s, hipEventCreate(&a));
+    HIPCHK(s, hipEventCreate(&b));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    HIPCHK(s, hipEventRecord(a, s->stream));
+    int rc = queue_steps(s, nsteps, false, false);
+    if (rc) return rc;
+    HIPCHK(s, hipEventRecord(b, s->stream));
+    HIPCHK(s, hipEventSynchronize(b));
+    float f = 0;
+    HIPCHK(s, hipEventElapsedTime(&f, a, b));
+    *ms = f;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return CNS_OK;
+}
+
+extern "C" int cns_set_sponge(cns_solver *s, const double *sigma_x, const double *sigma_z, const double *rref,
+                              const double *ruref, const double *rvref, const double *rwref, const double *reref) {
+    if (!s || !sigma_x || !sigma_z || !rref || !ruref || !rvref || !rwref || !reref)
+        return fail(s, CNS_ERR_ARG, "cns_set_sponge: null argument");
+    HIPCHK(s, hipSetDevice(s->dev));
+    const int mx = s->cfg.nx, mz = s->cfg.nz;
+    HIPCHK(s, hipMemcpy(s->d_spx, sigma_x, mx * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(s, hipMemcpy(s->d_spz, sigma_z, mz * sizeof(double), hipMemcpyHostToDevice));
+    const double *src[5] = {rref, ruref, rvref, rwref, reref};
+    for (int q = 0; q < 5; q++)
+        HIPCHK(s, hipMemcpy(s->d_ref[q], src[q], (size_t)mx * mz * sizeof(double), hipMemcpyHostToDevice));
+    return CNS_OK;
+}
+
+extern "C" int cns_get_scalars(cns_solver *s, double *dt, double *dpdz, double *time) {
+    if (!s) return fail(nullptr, CNS_ERR_ARG, "null solver");
+    HIPCHK(s, hipSetDevice(s->dev));
+    Scalars h{};
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    HIPCHK(s, hipMemcpy(&h, s->d_sc, sizeof(h), hipMemcpyDeviceToHost));
+    if (dt) *dt = h.dt;
+    if (dpdz) *dpdz = h.dpdz;
+    if (time) *time = h.time;
+    return CNS_OK;
+}
+
+extern "C" int cns_sync(cns_solver *s) {
+    if (!s) return fail(nullptr, CNS_ERR_ARG, "null solver");
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    return CNS_OK;
+}
+
+extern "C" int cns_profile_enable(cns_solver *s, int enable) {
+    if (!s) return fail(nullptr, CNS_ERR_ARG, "null solver");
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    s->prof = enable != 0;
+    s->prof_n = 0;
+    return CNS_OK;
+}
+
+extern "C" int cns_profile_read(cns_solver *s, double *total_ms, int *launches) {
+    if (!s || !total_ms || !launches) return fail(s, CNS_ERR_ARG, "cns_profile_read: null argument");
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    double tot = 0.0;
+    for (int k = 0; k < s->prof_n; k++) {
+        float f = 0;
+        HIPCHK(s, hipEventElapsedTime(&f, s->ev[2 * k], s->ev[2 * k + 1]));
+        tot += f;
+    }
+    *total_ms = tot;
+    *launches = s->prof_n;
+    return CNS_OK;
+}
+
+extern "C" int cns_device_bytes(const cns_solver *s, uint64_t *bytes) {
+    if (!s || !bytes) return fail(nullptr, CNS_ERR_ARG, "cns_device_bytes: null argument");
+    *bytes = s->dev_bytes;
+    return CNS_OK;
+}

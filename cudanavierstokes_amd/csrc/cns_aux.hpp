@@ -1,0 +1,195 @@
+// cns_aux.hpp -- pointwise and reduction kernels around the RHS:
+// state conversion (primitive <-> conserved), CFL time step, volume
+// integrals for the forcing controller and bulk statistics, and the
+// on-device step bookkeeping (dt, dp/dz, time) that replaces the
+// reference's host round trips (src/cuda_main.cu:249-265).
+#pragma once
+#include "cns_device.hpp"
+
+namespace cns {
+
+constexpr int RED_THREADS = 256;
+constexpr int RED_BLOCKS = 1024;  // fixed partial count: deterministic sums
+constexpr int NPART = 4;          // rho, rho*w, rho*E, u.u
+
+// Device-resident scalars of one solver.
+struct Scalars {
+    double dt, dpdz, time, pad;
+    unsigned long long conv_max, visc_max;
+    double part_out[NPART];
+};
+
+// (rho, u, v, w, rhoE) -> (rho, rho u, rho v, rho w, rhoE), in place
+template <typename R>
+__global__ void k_to_conserved(long n, R *r, R *u, R *v, R *w) {
+    for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x) {
+        const R rr = r[g];
+        u[g] = u[g] * rr;
+        v[g] = v[g] * rr;
+        w[g] = w[g] * rr;
+    }
+}
+
+// conserved -> primitive velocity into separate outputs
+template <typename R>
+__global__ void k_to_velocity(long n, const R *r, const R *ru, const R *rv, const R *rw, double *u, double *v,
+                              double *w) {
+    for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x) {
+        const R rr = r[g];
+        u[g] = (double)(ru[g] / rr);
+        v[g] = (double)(rv[g] / rr);
+        w[g] = (double)(rw[g] / rr);
+    }
+}
+
+__device__ __forceinline__ double dmax(double a, double b) { return a >= b ? a : b; }
+
+// Block maximum of two non-negative values, then one atomic max each.
+// Positive doubles order like their bit patterns, so an unsigned 64-bit
+// atomic max is exact.
+__device__ __forceinline__ void block_max2_to(double a, double b, unsigned long long *outa, unsigned long long *outb) {
+    __shared__ double ra[RED_THREADS / 64], rb[RED_THREADS / 64];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        a = fmax(a, __shfl_xor(a, off));
+        b = fmax(b, __shfl_xor(b, off));
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) {
+        ra[wid] = a;
+        rb[wid] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double ma = ra[0], mb = rb[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
+            ma = fmax(ma, ra[w]);
+            mb = fmax(mb, rb[w]);
+        }
+        atomicMax(outa, (unsigned long long)__double_as_longlong(ma));
+        if (outb) atomicMax(outb, (unsigned long long)__double_as_longlong(mb));
+    }
+}
+
+// Local CFL inverse time scales (src/calc_stress.cu:140-160):
+//   conv = max((|u|+c)/dx_i, (|v|+c)/dy, (|w|+c)/dz),  c = sqrt(g(g-1) e_int)
+//   visc = max(mu/dx_i^2, mu/dy^2, mu/dz^2)   (only with visc_from_state; the
+//   solver loop otherwise takes mu from the last RHS input, as the reference
+//   reads the mu field of the last calcState)
+template <typename R>
+__global__ void k_cfl(Params P, StagePtrs<R> A, int visc_from_state, Scalars *sc) {
+    const long mxy = (long)P.mx * P.my, n = mxy * P.mz;
+    double cmax = 0.0, vmax = 0.0;
+    for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(g % P.mx);
+        const double r = (double)A.qin[0][g];
+        const double u = (double)A.qin[1][g] / r, v = (double)A.qin[2][g] / r, w = (double)A.qin[3][g] / r;
+        const double ien = (double)A.qin[4][g] / r - 0.5 * (u * u + v * v + w * w);
+        const double sos = sqrt(P.gam * (P.gam - 1) * ien);
+        const double dx = P.dxv[i];
+        const double c1 = (fabs(u) + sos) / dx, c2 = (fabs(v) + sos) * P.d_dy, c3 = (fabs(w) + sos) * P.d_dz;
+        cmax = dmax(cmax, dmax(c1, dmax(c2, c3)));
+        if (visc_from_state) {
+            const double t = P.cvInv * ien;
+            const double mu = (P.unit_visc ? t : pow(t, P.viscexp)) / P.Re;
+            const double v1 = mu / (dx * dx), v2 = mu * P.d_d2y, v3 = mu * P.d_d2z;
+            vmax = dmax(vmax, dmax(v1, dmax(v2, v3)));
+        }
+    }
+    block_max2_to(cmax, vmax, &sc->conv_max, visc_from_state ? &sc->visc_max : nullptr);
+}
+
+// dt = min(1e5, CFL / max(conv, visc)), which equals the reference's
+// min over points of CFL / max(conv_i, visc_i) (division is monotone).
+__global__ void k_dt_finalize(double cfl, Scalars *sc) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const double cm = __longlong_as_double((long long)sc->conv_max);
+        const double vm = __longlong_as_double((long long)sc->visc_max);
+        const double m = cm >= vm ? cm : vm;
+        const double val = cfl / m;
+        sc->dt = (100000.0 <= val) ? 100000.0 : val;
+        sc->conv_max = 0ull;
+    }
+}
+
+__global__ void k_reset_visc(Scalars *sc) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) sc->visc_max = 0ull;
+}
+
+// time[istep] = time + dt (src/calc_stress.cu:12-14, src/cuda_math.cu:17-19)
+__global__ void k_advance_time(Scalars *sc, double *time_out, int istep) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        sc->time = sc->time + sc->dt;
+        if (time_out) time_out[istep] = sc->time;
+    }
+}
+
+// Per-block partial volume integrals, weight dxv[i]/d_dy/d_dz
+// (src/cuda_math.cu:143-171): rho, rho*w, rho*E, u.u.
+template <typename R>
+__global__ void k_partials(Params P, StagePtrs<R> A, double *part) {
+    const long mxy = (long)P.mx * P.my, n = mxy * P.mz;
+    double s[NPART] = {0, 0, 0, 0};
+    for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(g % P.mx);
+        const double wgt = P.dxv[i];
+        const double r = (double)A.qin[0][g];
+        const double ru = (double)A.qin[1][g], rv = (double)A.qin[2][g], rw = (double)A.qin[3][g];
+        const double u = ru / r, v = rv / r, w = rw / r;
+        s[0] += r * wgt / P.d_dy / P.d_dz;
+        s[1] += rw * wgt / P.d_dy / P.d_dz;
+        s[2] += (double)A.qin[4][g] * wgt / P.d_dy / P.d_dz;
+        s[3] += (u * u + v * v + w * w) * wgt / P.d_dy / P.d_dz;
+    }
+    __shared__ double sh[NPART][RED_THREADS];
+    for (int c = 0; c < NPART; c++) sh[c][threadIdx.x] = s[c];
+    __syncthreads();
+    for (int size = RED_THREADS / 2; size > 0; size /= 2) {
+        if ((int)threadIdx.x < size)
+            for (int c = 0; c < NPART; c++) sh[c][threadIdx.x] += sh[c][threadIdx.x + size];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int c = 0; c < NPART; c++) part[c * RED_BLOCKS + blockIdx.x] = sh[c][0];
+}
+
+// Final fixed-order sum of the partials, then either the pressure-gradient
+// controller (src/calc_stress.cu:16-18,98-120) or the bulk statistics
+// (src/calc_stress.cu:162-201).  what: 1 controller, 2 bulk.
+__global__ void k_final(const double *part, Scalars *sc, int what, int forcing, double vol, double *par1,
+                        double *par2, int istep) {
+    __shared__ double sh[NPART][RED_THREADS];
+    for (int c = 0; c < NPART; c++) {
+        double a = 0.0;
+        for (int b = threadIdx.x; b < RED_BLOCKS; b += RED_THREADS) a += part[c * RED_BLOCKS + b];
+        sh[c][threadIdx.x] = a;
+    }
+    __syncthreads();
+    for (int size = RED_THREADS / 2; size > 0; size /= 2) {
+        if ((int)threadIdx.x < size)
+            for (int c = 0; c < NPART; c++) sh[c][threadIdx.x] += sh[c][threadIdx.x + size];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        for (int c = 0; c < NPART; c++) sc->part_out[c] = sh[c][0];
+        const double rbulk = sh[0][0], rwint = sh[1][0], eint = sh[2][0], uu = sh[3][0];
+        if (what == 1) {
+            const double prev = sc->dpdz;
+            sc->dpdz = 0.99 * prev - 0.5 * (rwint / rbulk - 1);
+        } else if (what == 2) {
+            if (forcing) {
+                if (par1) par1[istep] = rwint / rbulk;
+                if (par2) par2[istep] = eint;
+            } else {
+                if (par1) par1[istep] = uu / vol;
+                if (par2) par2[istep] = __longlong_as_double(0x7ff8000000000000ll);
+            }
+        }
+    }
+}
+
+__global__ void k_fill(double *a, long n, double v) {
+    for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x) a[g] = v;
+}
+
+}  // namespace cns

@@ -1,0 +1,84 @@
+// cns_kernels.hip -- instantiates the RHS and dilatation kernels for one
+// advective half width S = CNS_S (compiled once per S), every viscous half
+// width SV <= S and both x-boundary / x-grid variants.
+#include "cns_launch.hpp"
+#include "cns_rhs.hpp"
+
+#ifndef CNS_S
+#error "compile with -DCNS_S=<1..4>"
+#endif
+
+namespace cns {
+
+template <typename R, int S, int SV>
+static hipError_t go_rhs(bool wallx, bool nonuni, int nb, hipStream_t st, const Params &P, const StagePtrs<R> &A) {
+    const dim3 g(nb), b(TILE_X * TILE_Y);
+    if (wallx) {
+        if (nonuni)
+            hipLaunchKernelGGL((k_rhs<R, S, SV, TILE_X, TILE_Y, true, true>), g, b, 0, st, P, A);
+        else
+            hipLaunchKernelGGL((k_rhs<R, S, SV, TILE_X, TILE_Y, true, false>), g, b, 0, st, P, A);
+    } else {
+        if (nonuni)
+            hipLaunchKernelGGL((k_rhs<R, S, SV, TILE_X, TILE_Y, false, true>), g, b, 0, st, P, A);
+        else
+            hipLaunchKernelGGL((k_rhs<R, S, SV, TILE_X, TILE_Y, false, false>), g, b, 0, st, P, A);
+    }
+    return hipGetLastError();
+}
+
+template <typename R, int SV>
+static hipError_t go_dil(bool wallx, bool nonuni, int nb, hipStream_t st, const Params &P, const StagePtrs<R> &A,
+                         R *dil) {
+    const dim3 g(nb), b(TILE_X * TILE_Y);
+    if (wallx) {
+        if (nonuni)
+            hipLaunchKernelGGL((k_dil<R, SV, TILE_X, TILE_Y, true, true>), g, b, 0, st, P, A, dil);
+        else
+            hipLaunchKernelGGL((k_dil<R, SV, TILE_X, TILE_Y, true, false>), g, b, 0, st, P, A, dil);
+    } else {
+        if (nonuni)
+            hipLaunchKernelGGL((k_dil<R, SV, TILE_X, TILE_Y, false, true>), g, b, 0, st, P, A, dil);
+        else
+            hipLaunchKernelGGL((k_dil<R, SV, TILE_X, TILE_Y, false, false>), g, b, 0, st, P, A, dil);
+    }
+    return hipGetLastError();
+}
+
+#define CNS_CAT2(a, b) a##b
+#define CNS_CAT(a, b) CNS_CAT2(a, b)
+#define LAUNCH_RHS CNS_CAT(launch_rhs_s, CNS_S)
+#define LAUNCH_DIL CNS_CAT(launch_dil_s, CNS_S)
+
+template <typename R>
+hipError_t LAUNCH_RHS(int sv, bool wallx, bool nonuni, int nb, hipStream_t st, const Params &P,
+                      const StagePtrs<R> &A) {
+    switch (sv) {
+#if CNS_S >= 1
+    case 1: return go_rhs<R, CNS_S, 1>(wallx, nonuni, nb, st, P, A);
+#endif
+#if CNS_S >= 2
+    case 2: return go_rhs<R, CNS_S, 2>(wallx, nonuni, nb, st, P, A);
+#endif
+#if CNS_S >= 3
+    case 3: return go_rhs<R, CNS_S, 3>(wallx, nonuni, nb, st, P, A);
+#endif
+#if CNS_S >= 4
+    case 4: return go_rhs<R, CNS_S, 4>(wallx, nonuni, nb, st, P, A);
+#endif
+    }
+    return hipErrorInvalidValue;
+}
+
+// here CNS_S is the viscous half width of the dilatation kernel
+template <typename R>
+hipError_t LAUNCH_DIL(bool wallx, bool nonuni, int nb, hipStream_t st, const Params &P, const StagePtrs<R> &A,
+                      R *dil) {
+    return go_dil<R, CNS_S>(wallx, nonuni, nb, st, P, A, dil);
+}
+
+template hipError_t LAUNCH_RHS<double>(int, bool, bool, int, hipStream_t, const Params &, const StagePtrs<double> &);
+template hipError_t LAUNCH_DIL<double>(bool, bool, int, hipStream_t, const Params &, const StagePtrs<double> &,
+                                       double *);
+
+}  // namespace cns

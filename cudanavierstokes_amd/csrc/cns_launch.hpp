@@ -1,0 +1,50 @@
+// cns_launch.hpp -- host-side launchers of the templated kernels.  Each
+// advective stencil half width S is compiled in its own translation unit
+// (cns_kernels.hip built with -DCNS_S=1..4).
+#pragma once
+#include "cns_device.hpp"
+
+namespace cns {
+
+constexpr int TILE_X = 64;
+constexpr int TILE_Y = 4;
+
+#define CNS_DECLARE_LAUNCH(N)                                                                                  \
+    template <typename R>                                                                                      \
+    hipError_t launch_rhs_s##N(int sv, bool wallx, bool nonuni, int nblocks, hipStream_t st, const Params &P, \
+                               const StagePtrs<R> &A);                                                         \
+    template <typename R>                                                                                      \
+    hipError_t launch_dil_s##N(bool wallx, bool nonuni, int nblocks, hipStream_t st, const Params &P,         \
+                               const StagePtrs<R> &A, R *dil);
+CNS_DECLARE_LAUNCH(1)
+CNS_DECLARE_LAUNCH(2)
+CNS_DECLARE_LAUNCH(3)
+CNS_DECLARE_LAUNCH(4)
+#undef CNS_DECLARE_LAUNCH
+
+template <typename R>
+inline hipError_t launch_rhs(int s, int sv, bool wallx, bool nonuni, int nblocks, hipStream_t st, const Params &P,
+                             const StagePtrs<R> &A) {
+    switch (s) {
+    case 1: return launch_rhs_s1<R>(sv, wallx, nonuni, nblocks, st, P, A);
+    case 2: return launch_rhs_s2<R>(sv, wallx, nonuni, nblocks, st, P, A);
+    case 3: return launch_rhs_s3<R>(sv, wallx, nonuni, nblocks, st, P, A);
+    case 4: return launch_rhs_s4<R>(sv, wallx, nonuni, nblocks, st, P, A);
+    }
+    return hipErrorInvalidValue;
+}
+
+// the dilatation kernel only depends on the viscous half width
+template <typename R>
+inline hipError_t launch_dil(int sv, bool wallx, bool nonuni, int nblocks, hipStream_t st, const Params &P,
+                             const StagePtrs<R> &A, R *dil) {
+    switch (sv) {
+    case 1: return launch_dil_s1<R>(wallx, nonuni, nblocks, st, P, A, dil);
+    case 2: return launch_dil_s2<R>(wallx, nonuni, nblocks, st, P, A, dil);
+    case 3: return launch_dil_s3<R>(wallx, nonuni, nblocks, st, P, A, dil);
+    case 4: return launch_dil_s4<R>(wallx, nonuni, nblocks, st, P, A, dil);
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace cns

@@ -1,0 +1,252 @@
+"""Host-side mirror of the reference solver interface over libcns.so.
+
+Reference entry points (src/main.h:23-42) and their counterparts here:
+
+  setDevice(int)                 -> setDevice(dev)           (config.device)
+  setGPUParameters(Communicator) -> setGPUParameters(config) (grid, metrics, coefficients)
+  initSolver(Communicator)       -> initSolver()             (device buffers)
+  copyField(direction, ...)      -> copyField(0, fields) host->device, copyField(1) device->host
+  runSimulation(kin, enst, time) -> runSimulation(nsteps)    (returns kin, enst, time)
+  clearSolver(Communicator)      -> clearSolver()
+
+Solver is the handle-based form of the same interface; several instances can
+coexist, unlike the reference's compile-time globals.
+"""
+import ctypes as C
+from dataclasses import dataclass, fields
+
+import numpy as np
+
+from . import _lib
+from ._lib import CnsConfig, CnsError, check
+
+SCHEMES = {"rk3": 0, "smr_rk3": 0, "lowstorage": 0, "kutta_rk3": 1, "rk4": 2}
+
+
+@dataclass
+class Config:
+    """Run-time counterpart of the reference's case macros (src/globals.h)."""
+    mx: int = 64
+    my: int = 64
+    mz: int = 64
+    stencil_size: int = 2
+    stencil_visc: int = 2
+    periodic_x: int = 1
+    nonuniform_x: int = 0
+    forcing: int = 0
+    sponge: int = 0
+    scheme: int = 0
+    check_cfl: int = 10
+    check_bulk: int = 10
+    fix_y_work: int = 0
+    precision: int = 0
+    device: int = 0
+    Lx: float = 2 * np.pi
+    Ly: float = 2 * np.pi
+    Lz: float = 2 * np.pi
+    Re: float = 1600.0
+    Pr: float = 0.71
+    Ma: float = 0.1
+    gam: float = 1.4
+    viscexp: float = 1.0
+    cfl: float = 0.5
+    stretch: float = 1.8
+    twall_bot: float = 1.0
+    twall_top: float = 1.0
+    dpdz0: float = 0.00372
+
+    def to_c(self):
+        c = CnsConfig()
+        for f in fields(self):
+            v = getattr(self, f.name)
+            if f.name == "scheme" and isinstance(v, str):
+                v = SCHEMES[v]
+            setattr(c, _lib.C_NAMES.get(f.name, f.name), v)
+        return c
+
+    def replace(self, **kw):
+        d = {f.name: getattr(self, f.name) for f in fields(self)}
+        d.update(kw)
+        return Config(**d)
+
+
+def taylor_green(n=64, order=4, scheme="rk3", **kw):
+    """Periodic Taylor-Green box of side 2*pi (configs of src/init.cpp:126-148)."""
+    s = order // 2
+    return Config(mx=n, my=n, mz=n, stencil_size=s, stencil_visc=s, scheme=SCHEMES.get(scheme, scheme), **kw)
+
+
+def channel(mx=160, my=128, mz=192, order=6, scheme="rk3", **kw):
+    """Isothermal-wall channel: x wall-normal (stretched, walls), y spanwise,
+    z streamwise with the bulk-velocity controller (globals/channel.h)."""
+    s = order // 2
+    d = dict(mx=mx, my=my, mz=mz, stencil_size=s, stencil_visc=s, periodic_x=0, nonuniform_x=1,
+             forcing=1, scheme=SCHEMES.get(scheme, scheme), Lx=2.0, Ly=4 * np.pi / 3, Lz=4 * np.pi,
+             Re=3000.0, Pr=0.72, Ma=1.5, viscexp=0.7, stretch=1.8)
+    d.update(kw)
+    return Config(**d)
+
+
+def _f64(a, n):
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(-1))
+    if a.size != n:
+        raise ValueError(f"expected {n} values, got {a.size}")
+    return a
+
+
+class Solver:
+    """One flow instance on one GPU (a cns_solver handle)."""
+
+    def __init__(self, config=None, **kw):
+        self.config = config if config is not None else Config(**kw)
+        self.L = _lib.load()
+        self._c = self.config.to_c()
+        h = C.c_void_p()
+        check(self.L.cns_create(C.byref(self._c), C.byref(h)), None, "cns_create")
+        self.h = h
+        c = self.config
+        self.shape = (c.mz, c.my, c.mx)
+        self.n = c.mx * c.my * c.mz
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.cns_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _chk(self, rc, what):
+        check(rc, self.h, what)
+
+    def grid(self):
+        c = self.config
+        x, xp, xpp, dxv = (np.empty(c.mx) for _ in range(4))
+        y, z = np.empty(c.my), np.empty(c.mz)
+        self._chk(self.L.cns_get_grid(self.h, x.ctypes.data, y.ctypes.data, z.ctypes.data, xp.ctypes.data,
+                                      xpp.ctypes.data, dxv.ctypes.data), "cns_get_grid")
+        return dict(x=x, y=y, z=z, xp=xp, xpp=xpp, dxv=dxv)
+
+    def taylor_green_fields(self):
+        out = [np.empty(self.n) for _ in range(5)]
+        self._chk(self.L.cns_fill_taylor_green(self.h, *out), "cns_fill_taylor_green")
+        return [a.reshape(self.shape) for a in out]
+
+    def channel_fields(self, seed=1, ubulk=1.0, amp=0.1):
+        out = [np.empty(self.n) for _ in range(5)]
+        self._chk(self.L.cns_fill_channel(self.h, seed, ubulk, amp, *out), "cns_fill_channel")
+        return [a.reshape(self.shape) for a in out]
+
+    def set_state(self, r, u, v, w, e):
+        self._chk(self.L.cns_set_state(self.h, *[_f64(a, self.n) for a in (r, u, v, w, e)]), "cns_set_state")
+
+    def get_state(self):
+        out = [np.empty(self.n) for _ in range(5)]
+        self._chk(self.L.cns_get_state(self.h, *out), "cns_get_state")
+        return [a.reshape(self.shape) for a in out]
+
+    def rhs(self):
+        out = [np.empty(self.n) for _ in range(5)]
+        self._chk(self.L.cns_rhs(self.h, *out), "cns_rhs")
+        return [a.reshape(self.shape) for a in out]
+
+    def step(self, nsteps, outputs=True):
+        if not outputs:
+            self._chk(self.L.cns_step(self.h, nsteps, None, None, None), "cns_step")
+            return None
+        k, e, t = np.empty(nsteps), np.empty(nsteps), np.empty(nsteps)
+        self._chk(self.L.cns_step(self.h, nsteps, k.ctypes.data, e.ctypes.data, t.ctypes.data), "cns_step")
+        return k, e, t
+
+    def set_sponge(self, sigma_x, sigma_z, refs):
+        c = self.config
+        self._chk(self.L.cns_set_sponge(self.h, _f64(sigma_x, c.mx), _f64(sigma_z, c.mz),
+                                        *[_f64(a, c.mx * c.mz) for a in refs]), "cns_set_sponge")
+
+    def scalars(self):
+        dt, dp, t = C.c_double(), C.c_double(), C.c_double()
+        self._chk(self.L.cns_get_scalars(self.h, C.byref(dt), C.byref(dp), C.byref(t)), "cns_get_scalars")
+        return dict(dt=dt.value, dpdz=dp.value, time=t.value)
+
+    def sync(self):
+        self._chk(self.L.cns_sync(self.h), "cns_sync")
+
+    def profile_enable(self, on=True):
+        self._chk(self.L.cns_profile_enable(self.h, int(on)), "cns_profile_enable")
+
+    def profile_read(self):
+        ms, n = C.c_double(), C.c_int()
+        self._chk(self.L.cns_profile_read(self.h, C.byref(ms), C.byref(n)), "cns_profile_read")
+        return ms.value, n.value
+
+    def time_steps(self, nsteps):
+        ms = C.c_double()
+        self._chk(self.L.cns_time_steps(self.h, nsteps, C.byref(ms)), "cns_time_steps")
+        return ms.value
+
+    def device_bytes(self):
+        b = C.c_uint64()
+        self._chk(self.L.cns_device_bytes(self.h, C.byref(b)), "cns_device_bytes")
+        return b.value
+
+
+# ---------------------------------------------------------------------------
+# Reference-named module functions over one default solver (src/main.h:23-42)
+# ---------------------------------------------------------------------------
+_default = {"device": 0, "config": None, "solver": None}
+
+
+def setDevice(device):
+    _default["device"] = int(device)
+
+
+def setGPUParameters(config):
+    _default["config"] = config.replace(device=_default["device"])
+
+
+def initSolver():
+    if _default["config"] is None:
+        raise CnsError("initSolver: call setGPUParameters first")
+    if _default["solver"] is not None:
+        _default["solver"].close()
+    _default["solver"] = Solver(_default["config"])
+    return _default["solver"]
+
+
+def _solver():
+    s = _default["solver"]
+    if s is None:
+        raise CnsError("initSolver has not been called")
+    return s
+
+
+def copyField(direction, fields_=None):
+    """direction 0: host (r,u,v,w,e) -> device; 1: device -> host, returned."""
+    s = _solver()
+    if direction == 0:
+        s.set_state(*fields_)
+        return None
+    if direction == 1:
+        return s.get_state()
+    raise ValueError("direction must be 0 or 1")
+
+
+def runSimulation(nsteps):
+    """nsteps steps of the configured scheme; returns (kin, enst, time)."""
+    return _solver().step(nsteps)
+
+
+def clearSolver():
+    s = _default["solver"]
+    if s is not None:
+        s.close()
+    _default["solver"] = None

@@ -1,0 +1,141 @@
+/*
+ * cns.h -- C ABI of the MI355X (gfx950) explicit compressible Navier-Stokes
+ * time advance: right-hand side, ghost fills, sponge and RK stage updates of
+ * simone-silvestri/CudaNavierStokes, as a drop-in for its solver entry
+ * points.  Plain pointers and sizes only.
+ *
+ * Conventions
+ *  - Fields are float64 host arrays in the reference layout
+ *    idx = i + j*mx + k*mx*my, x unit-stride (src/globals.h:60-61).
+ *  - State variables are the reference's (rho, u, v, w, rhoE): density,
+ *    velocity, total energy per volume (src/init.cpp:139-146).
+ *  - Axes follow the reference: x may be stretched and carry isothermal walls,
+ *    y and z are periodic, the forcing acts along z (src/cuda_rhs.cu:382-394).
+ *  - Names avoid the reference's case macros (mx, Lx, Re, forcing, nsteps,
+ *    ...), so this header can be included after src/globals.h.
+ *  - Every function returns CNS_OK (0) or a negative error code; the message
+ *    of the last failure is cns_last_error(handle) (or cns_last_error(NULL)
+ *    when no handle exists).  Nothing calls exit().
+ *  - The library owns device memory; the caller owns host buffers.  A handle
+ *    is used by one host thread at a time; handles are independent.
+ */
+#ifndef CNS_H_
+#define CNS_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CNS_OK 0
+#define CNS_ERR_ARG (-1)
+#define CNS_ERR_HIP (-2)
+#define CNS_ERR_STATE (-3)
+#define CNS_ERR_UNSUPPORTED (-4)
+#define CNS_ERR_NOMEM (-5)
+
+/* time-advance schemes */
+#define CNS_RK3_LOWSTORAGE 0 /* Spalart-Moser-Rogers RK3, src/cuda_main.cu:109-186 */
+#define CNS_RK3_KUTTA 1      /* Kutta RK3, src/cuda_main.cu:44-107 */
+#define CNS_RK4 2            /* classic RK4 (no reference counterpart) */
+
+/* storage precision of the conserved variables */
+#define CNS_FP64 0
+#define CNS_FP32 1
+
+/* Run-time form of the reference's compile-time case macros
+ * (src/globals.h:14-58, src/sponge.h:4-17). */
+typedef struct cns_config {
+    int nx, ny, nz;       /* grid points (mx_tot, my_tot, mz_tot) */
+    int stencil_size;     /* advective half width s: order 2s (stencilSize) */
+    int stencil_visc;     /* viscous half width sv <= s (stencilVisc) */
+    int periodic_x;       /* periodicX */
+    int nonuniform_x;     /* nonUniformX: tanh-stretched x with metrics */
+    int forcing_z;        /* forcing: dp/dz controller driving bulk velocity 1 */
+    int sponge;           /* sponge relaxation (boundaryLayer's addSponge) */
+    int scheme;           /* CNS_RK3_LOWSTORAGE (lowStorage), CNS_RK3_KUTTA, CNS_RK4 */
+    int check_cfl;        /* checkCFLcondition */
+    int check_bulk;       /* checkBulk */
+    int fix_y_work;       /* 0: y viscous work exactly as src/cuda_rhs.cu:175; 1: corrected */
+    int precision;        /* CNS_FP64 or CNS_FP32 */
+    int device;           /* HIP device ordinal */
+    double len_x, len_y, len_z;  /* Lx, Ly, Lz */
+    double reynolds, prandtl, mach, gamma; /* Re, Pr, Ma, gam */
+    double visc_exp;      /* viscexp: mu = T^visc_exp / Re */
+    double cfl;           /* CFL */
+    double stretch;       /* tanh stretching of x (stretch) */
+    double twall_bot, twall_top; /* wall temperatures (TwallBot, TwallTop) */
+    double dpdz0;         /* initial forcing (src/cuda_utils.cu:68-70) */
+} cns_config;
+
+typedef struct cns_solver cns_solver;
+
+/* Defaults: 64^3 periodic box of side 2*pi, s = sv = 2, SMR RK3, Re 1600,
+ * Pr 0.71, Ma 0.1, gamma 1.4, viscexp 1, CFL 0.5, checks every 10 steps. */
+int cns_config_init(cns_config *cfg);
+
+/* replaces setGPUParameters + initSolver (src/cuda_utils.cu:49,415) */
+int cns_create(const cns_config *cfg, cns_solver **out);
+/* replaces clearSolver (src/cuda_utils.cu:521) */
+int cns_destroy(cns_solver *s);
+const char *cns_last_error(const cns_solver *s);
+
+/* grid of initGrid/derivGrid (src/init.cpp:32-92,258-277); any pointer may
+ * be NULL.  x, xp (dxi/dx), xpp (d2x/dxi2), dxv (cell widths) have mx
+ * entries, y my, z mz. */
+int cns_get_grid(const cns_solver *s, double *x, double *y, double *z, double *xp, double *xpp, double *dxv);
+
+/* Initial conditions written into host arrays (src/init.cpp:94-148):
+ * Taylor-Green vortex of initCHIT; compressible channel with a laminar
+ * Poiseuille profile and SplitMix64 uniform perturbations of amplitude
+ * amp*ubulk on every velocity component. */
+int cns_fill_taylor_green(const cns_solver *s, double *r, double *u, double *v, double *w, double *e);
+int cns_fill_channel(const cns_solver *s, uint64_t seed, double ubulk, double amp, double *r, double *u,
+                     double *v, double *w, double *e);
+
+/* replaces copyField(0): host (rho,u,v,w,rhoE) -> device; resets time */
+int cns_set_state(cns_solver *s, const double *r, const double *u, const double *v, const double *w,
+                  const double *e);
+/* replaces copyField(1): device -> host (rho,u,v,w,rhoE) */
+int cns_get_state(cns_solver *s, double *r, double *u, double *v, double *w, double *e);
+
+/* one right-hand-side evaluation of the current state (calcRHS,
+ * src/cuda_main.cu:15-42), five host arrays of the conserved-variable RHS */
+int cns_rhs(cns_solver *s, double *rr, double *ru, double *rv, double *rw, double *re);
+
+/* replaces runSimulation[LowStorage] (src/cuda_main.cu:44-186): n_steps
+ * steps.  Optional host outputs of length n_steps: kin/enst receive the
+ * reference's par1/par2 bulk statistics at steps with istep % check_bulk == 0
+ * (NaN elsewhere; par2 is NaN without forcing), time the simulation time
+ * after each step. */
+int cns_step(cns_solver *s, int n_steps, double *kin, double *enst, double *time);
+
+/* sponge of addSponge (src/sponge.cu:31-41): strengths sigma_x[mx],
+ * sigma_z[mz] and reference conserved fields [mz][mx] */
+int cns_set_sponge(cns_solver *s, const double *sigma_x, const double *sigma_z, const double *rref,
+                   const double *ruref, const double *rvref, const double *rwref, const double *reref);
+
+/* current dt, dp/dz and simulation time */
+int cns_get_scalars(cns_solver *s, double *dt, double *dpdz, double *time);
+
+/* Wait for all work queued on the solver's stream. */
+int cns_sync(cns_solver *s);
+
+/* Bracket every fused RHS/stage-update kernel launch with HIP events on the
+ * solver's stream (enable=1); cns_profile_read returns the summed kernel
+ * time in ms and the number of launches since the last enable. */
+int cns_profile_enable(cns_solver *s, int enable);
+int cns_profile_read(cns_solver *s, double *total_ms, int *launches);
+
+/* Steps with inputs resident on the device, timed with HIP events on the
+ * solver's stream; *ms receives the elapsed time of the n_steps. */
+int cns_time_steps(cns_solver *s, int n_steps, double *ms);
+
+/* bytes of device memory held by the solver */
+int cns_device_bytes(const cns_solver *s, uint64_t *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CNS_H_ */

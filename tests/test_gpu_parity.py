@@ -1,0 +1,141 @@
+"""GPU parity: the HIP path (libcns.so through its C ABI) against the FP64
+oracle on identical inputs.
+
+Tolerances (relative, per conserved variable; momenta normalised jointly):
+  one RHS evaluation        L2 <= 1e-12, Linf <= 1e-10
+  state after N steps       L2 <= 1e-12, Linf <= 1e-10
+The GPU evaluates the same discrete operators in a different floating-point
+order (fused multiply-add, telescoped flux pairs), so agreement is to
+rounding, not bitwise.
+"""
+import numpy as np
+import pytest
+
+import cudanavierstokes_amd as cns
+from tests import cases
+from tests.helpers import assert_close, flush_print, initial_fields, oracle_from, state_errors
+
+pytestmark = pytest.mark.gpu
+
+RHS_L2, RHS_LINF = 1e-12, 1e-10
+ST_L2, ST_LINF = 1e-12, 1e-10
+
+
+def _pair(cfg, kind, seed=1):
+    flush_print(f"[case] {kind} {cfg}")
+    orc = oracle_from(cfg)
+    f = initial_fields(kind, orc, seed=seed)
+    orc.set_state(*f)
+    sol = cns.Solver(cfg)
+    sol.set_state(*f)
+    return orc, sol, f
+
+
+RHS_CASES = [
+    ("tgv_s1", "tgv", dict(mx=16, my=16, mz=16, stencil_size=1, stencil_visc=1)),
+    ("tgv_s2", "tgv", dict(mx=16, my=16, mz=16, stencil_size=2, stencil_visc=2)),
+    ("tgv_s3", "tgv", dict(mx=24, my=24, mz=24, stencil_size=3, stencil_visc=3)),
+    ("tgv_s4", "tgv", dict(mx=24, my=24, mz=24, stencil_size=4, stencil_visc=4)),
+    ("tgv_s3_sv2", "tgv", dict(mx=16, my=16, mz=16, stencil_size=3, stencil_visc=2)),
+    ("random_box", "random", dict(mx=40, my=20, mz=12, stencil_size=2, stencil_visc=2, Lx=3.0, Ly=2.0, Lz=1.5,
+                                  viscexp=0.7, Ma=0.4, Re=300.0)),
+    ("random_box_s4_fix", "random", dict(mx=72, my=17, mz=13, stencil_size=4, stencil_visc=4, viscexp=0.76,
+                                         fix_y_work=1, Ma=0.3, Re=500.0)),
+    ("channel_s3", "channel", dict(mx=24, my=16, mz=20, stencil_size=3, stencil_visc=3, periodic_x=0,
+                                   nonuniform_x=1, forcing=1, Lx=2.0, Ly=4 * np.pi / 3, Lz=4 * np.pi,
+                                   Re=3000.0, Pr=0.72, Ma=1.5, viscexp=0.7)),
+    ("channel_s4_uniform_wall", "channel", dict(mx=70, my=12, mz=16, stencil_size=4, stencil_visc=4,
+                                                periodic_x=0, nonuniform_x=0, Lx=2.0, Ma=0.8, viscexp=0.7,
+                                                twall_bot=1.1, twall_top=0.9)),
+    ("periodic_stretched", "random", dict(mx=20, my=16, mz=16, stencil_size=2, stencil_visc=1, nonuniform_x=1,
+                                          Lx=2.0, Ma=0.5)),
+]
+
+
+@pytest.mark.parametrize("name,kind,kw", RHS_CASES, ids=[c[0] for c in RHS_CASES])
+def test_rhs_parity(name, kind, kw):
+    cfg = cns.Config(**kw)
+    orc, sol, _ = _pair(cfg, kind)
+    ro = orc.rhs()
+    rg = sol.rhs()
+    errs = cases.rel_errors(rg, ro)
+    flush_print(name, errs)
+    assert_close(errs, RHS_L2, RHS_LINF, f"rhs {name}")
+    sol.close()
+
+
+def test_grid_matches_oracle():
+    for kw in (dict(mx=24, my=16, mz=20, nonuniform_x=1, periodic_x=0, Lx=2.0, stretch=1.8, stencil_size=3,
+                    stencil_visc=3),
+               dict(mx=16, my=12, mz=10, stencil_size=4, stencil_visc=2)):
+        cfg = cns.Config(**kw)
+        orc = oracle_from(cfg)
+        sol = cns.Solver(cfg)
+        go, gs = orc.grid(), sol.grid()
+        for k in ("x", "y", "z", "xp", "xpp", "dxv"):
+            assert np.array_equal(go[k], gs[k]), k
+        sol.close()
+
+
+def test_sponge_parity():
+    cfg = cns.Config(mx=20, my=12, mz=16, stencil_size=2, stencil_visc=2, sponge=1, Ma=0.3)
+    orc, sol, f = _pair(cfg, "random", seed=3)
+    rng = np.random.default_rng(5)
+    sx, sz = rng.uniform(0, 2, cfg.mx), rng.uniform(0, 1, cfg.mz)
+    refs = [rng.uniform(0.5, 1.5, (cfg.mz, cfg.mx)) for _ in range(5)]
+    orc.set_sponge(sx, sz, refs)
+    sol.set_sponge(sx, sz, refs)
+    errs = cases.rel_errors(sol.rhs(), orc.rhs())
+    assert_close(errs, RHS_L2, RHS_LINF, "sponge")
+    sol.close()
+
+
+STEP_CASES = [
+    ("tgv16_smr", "tgv", dict(mx=16, my=16, mz=16, stencil_size=2, stencil_visc=2, scheme=0, check_cfl=3,
+                              check_bulk=2), 12),
+    ("tgv16_kutta", "tgv", dict(mx=16, my=16, mz=16, stencil_size=2, stencil_visc=2, scheme=1), 6),
+    ("tgv24_rk4_s4", "tgv", dict(mx=24, my=24, mz=24, stencil_size=4, stencil_visc=4, scheme=2), 6),
+    ("channel_smr", "channel", dict(mx=24, my=16, mz=20, stencil_size=3, stencil_visc=3, periodic_x=0,
+                                    nonuniform_x=1, forcing=1, Lx=2.0, Ly=4 * np.pi / 3, Lz=4 * np.pi,
+                                    Re=3000.0, Pr=0.72, Ma=1.5, viscexp=0.7, check_cfl=4, check_bulk=4), 10),
+]
+
+
+@pytest.mark.parametrize("name,kind,kw,nsteps", STEP_CASES, ids=[c[0] for c in STEP_CASES])
+def test_step_parity(name, kind, kw, nsteps):
+    cfg = cns.Config(**kw)
+    orc, sol, _ = _pair(cfg, kind)
+    p1o, p2o, to = orc.step(nsteps)
+    p1g, p2g, tg = sol.step(nsteps)
+    so = orc.get_state()
+    sg = sol.get_state()
+    errs = state_errors(sg, so)
+    flush_print(name, errs)
+    assert_close(errs, ST_L2, ST_LINF, f"state {name}")
+    np.testing.assert_allclose(tg, to, rtol=1e-13, atol=0)
+    m = ~np.isnan(p1o)
+    assert np.array_equal(m, ~np.isnan(p1g))
+    np.testing.assert_allclose(p1g[m], p1o[m], rtol=1e-12)
+    m2 = ~np.isnan(p2o)
+    assert np.array_equal(m2, ~np.isnan(p2g))
+    np.testing.assert_allclose(p2g[m2], p2o[m2], rtol=1e-12)
+    sc = sol.scalars()
+    assert sc["dt"] == pytest.approx(orc.dt, rel=1e-13)
+    assert sc["dpdz"] == pytest.approx(orc.dpdz, rel=1e-11, abs=1e-15)
+    sol.close()
+
+
+def test_determinism():
+    cfg = cns.Config(mx=24, my=24, mz=24, stencil_size=3, stencil_visc=3)
+    outs = []
+    for _ in range(2):
+        orc = oracle_from(cfg)
+        f = initial_fields("tgv", orc)
+        sol = cns.Solver(cfg)
+        sol.set_state(*f)
+        k, _, t = sol.step(5)
+        outs.append((sol.get_state(), k, t))
+        sol.close()
+    for a, b in zip(outs[0][0], outs[1][0]):
+        assert np.array_equal(a, b)
+    assert np.array_equal(outs[0][1], outs[1][1], equal_nan=True)
