@@ -103,13 +103,13 @@ struct cns_solver {
     double cF[4] = {0}, cS[5] = {0}, cVF[4] = {0}, cVS[5] = {0};
     // device
     double *d_xp = nullptr, *d_vsx = nullptr, *d_dxv = nullptr, *d_spx = nullptr, *d_spz = nullptr;
-    double *d_ref[5] = {nullptr};
+    double *d_ref = nullptr;  // 5 sponge reference fields, mx*mz each
     Scalars *d_sc = nullptr;
     double *d_part = nullptr;
     double *d_dil = nullptr;
     int nsets = 0, nrhs = 0;
-    double *sets[4][5] = {{nullptr}};
-    double *rbuf[2][5] = {{nullptr}};
+    double *sets[4] = {nullptr};  // conserved-field sets, 5*n each
+    double *rbuf[2] = {nullptr};  // right-hand-side sets, 5*n each
     int cur = 0;
     double *d_par1 = nullptr, *d_par2 = nullptr, *d_time = nullptr;
     int par_cap = 0;
@@ -287,7 +287,7 @@ static void fill_params(cns_solver *s) {
     P.dxv = s->d_dxv;
     P.spx = s->d_spx;
     P.spz = s->d_spz;
-    for (int q = 0; q < 5; q++) P.ref[q] = s->d_ref[q];
+    P.ref = s->d_ref;
     P.dt = &s->d_sc->dt;
     P.dpdz = &s->d_sc->dpdz;
     P.visc_max_out = &s->d_sc->visc_max;
@@ -375,14 +375,14 @@ extern "C" int cns_destroy(cns_solver *s) {
     fr(s->d_dxv);
     fr(s->d_spx);
     fr(s->d_spz);
-    for (int q = 0; q < 5; q++) fr(s->d_ref[q]);
+    fr(s->d_ref);
     if (s->d_sc) (void)hipFree(s->d_sc);
     fr(s->d_part);
     fr(s->d_dil);
     for (int a = 0; a < 4; a++)
-        for (int q = 0; q < 5; q++) fr(s->sets[a][q]);
+        fr(s->sets[a]);
     for (int a = 0; a < 2; a++)
-        for (int q = 0; q < 5; q++) fr(s->rbuf[a][q]);
+        fr(s->rbuf[a]);
     fr(s->d_par1);
     fr(s->d_par2);
     fr(s->d_time);
@@ -441,14 +441,11 @@ extern "C" int cns_create(const cns_config *cfg, cns_solver **out) {
         (rc = dalloc(s, &s->d_dxv, mx)) || (rc = dalloc(s, &s->d_spx, mx)) || (rc = dalloc(s, &s->d_spz, mz)) ||
         (rc = dalloc(s, &s->d_part, (size_t)NPART * RED_BLOCKS)) || (rc = dalloc(s, &s->d_dil, n)))
         return bail(rc);
-    for (int q = 0; q < 5; q++)
-        if ((rc = dalloc(s, &s->d_ref[q], (size_t)mx * mz))) return bail(rc);
+    if ((rc = dalloc(s, &s->d_ref, (size_t)5 * mx * mz))) return bail(rc);
     for (int a = 0; a < s->nsets; a++)
-        for (int q = 0; q < 5; q++)
-            if ((rc = dalloc(s, &s->sets[a][q], n))) return bail(rc);
+        if ((rc = dalloc(s, &s->sets[a], 5 * (size_t)n))) return bail(rc);
     for (int a = 0; a < s->nrhs; a++)
-        for (int q = 0; q < 5; q++)
-            if ((rc = dalloc(s, &s->rbuf[a][q], n))) return bail(rc);
+        if ((rc = dalloc(s, &s->rbuf[a], 5 * (size_t)n))) return bail(rc);
     if (hipMalloc((void **)&s->d_sc, sizeof(Scalars)) != hipSuccess) {
         s->err = "hipMalloc(scalars) failed";
         return bail(CNS_ERR_NOMEM);
@@ -471,8 +468,8 @@ extern "C" int cns_create(const cns_config *cfg, cns_solver **out) {
         return bail(CNS_ERR_HIP);
     }
     for (int q = 0; q < 5; q++)
-        if (hipMemcpy(s->d_ref[q], zeros.data(), (size_t)mx * mz * sizeof(double), hipMemcpyHostToDevice) !=
-            hipSuccess) {
+        if (hipMemcpy(s->d_ref + (size_t)q * mx * mz, zeros.data(), (size_t)mx * mz * sizeof(double),
+                      hipMemcpyHostToDevice) != hipSuccess) {
             s->err = "sponge initialisation failed";
             return bail(CNS_ERR_HIP);
         }
@@ -548,21 +545,20 @@ extern "C" int cns_fill_channel(const cns_solver *s, uint64_t seed, double ubulk
 // Pointers a stage does not use still point at valid device memory (the
 // stage input), so no kernel argument is ever a null device address.
 static void fill_unused(StagePtrs<double> &A) {
-    for (int q = 0; q < 5; q++) {
-        double *in = const_cast<double *>(A.qin[q]);
-        if (!A.qout[q]) A.qout[q] = in;
-        if (!A.rout[q]) A.rout[q] = in;
-        if (!A.acc[q]) A.acc[q] = in;
-        if (!A.ra[q]) A.ra[q] = in;
-        if (!A.rb[q]) A.rb[q] = in;
-        if (!A.q0[q]) A.q0[q] = in;
-    }
+    double *in = const_cast<double *>(A.qin);
+    if (!A.qout) A.qout = in;
+    if (!A.rout) A.rout = in;
+    if (!A.acc) A.acc = in;
+    if (!A.ra) A.ra = in;
+    if (!A.rb) A.rb = in;
+    if (!A.q0) A.q0 = in;
 }
 
 static StagePtrs<double> ptrs_in(cns_solver *s, int set) {
     StagePtrs<double> A;
     std::memset(&A, 0, sizeof(A));
-    for (int q = 0; q < 5; q++) A.qin[q] = s->sets[set][q];
+    A.qin = s->sets[set];
+    A.fs = s->n;
     A.dil = s->d_dil;
     fill_unused(A);
     return A;
@@ -575,9 +571,10 @@ extern "C" int cns_set_state(cns_solver *s, const double *r, const double *u, co
     const size_t b = s->n * sizeof(double);
     const double *src[5] = {r, u, v, w, e};
     s->cur = 0;
-    for (int q = 0; q < 5; q++) HIPCHK(s, hipMemcpyAsync(s->sets[0][q], src[q], b, hipMemcpyHostToDevice, s->stream));
-    hipLaunchKernelGGL(k_to_conserved<double>, dim3(1024), dim3(256), 0, s->stream, s->n, s->sets[0][0],
-                       s->sets[0][1], s->sets[0][2], s->sets[0][3]);
+    for (int q = 0; q < 5; q++)
+        HIPCHK(s, hipMemcpyAsync(s->sets[0] + q * s->n, src[q], b, hipMemcpyHostToDevice, s->stream));
+    hipLaunchKernelGGL(k_to_conserved<double>, dim3(1024), dim3(256), 0, s->stream, s->n, s->sets[0],
+                       s->sets[0] + s->n, s->sets[0] + 2 * s->n, s->sets[0] + 3 * s->n);
     LAUNCHED(s, "k_to_conserved<double>");
     // time restarts at zero (initDevice, src/cuda_utils.cu:386)
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(1), 0, s->stream, &s->d_sc->time, 1L, 0.0);
@@ -593,9 +590,11 @@ extern "C" int cns_get_state(cns_solver *s, double *r, double *u, double *v, dou
     if (!s->have_state) return fail(s, CNS_ERR_STATE, "cns_get_state: no state set");
     HIPCHK(s, hipSetDevice(s->dev));
     const size_t b = s->n * sizeof(double);
-    double *const *q = s->sets[s->cur];
+    const long n = s->n;
+    double *const set = s->sets[s->cur];
+    double *const q[5] = {set, set + n, set + 2 * n, set + 3 * n, set + 4 * n};
     // velocities into the dilatation buffer and the first rhs buffer
-    double *tu = s->rbuf[0][0], *tv = s->rbuf[0][1], *tw = s->rbuf[0][2];
+    double *tu = s->rbuf[0], *tv = s->rbuf[0] + n, *tw = s->rbuf[0] + 2 * n;
     hipLaunchKernelGGL(k_to_velocity<double>, dim3(1024), dim3(256), 0, s->stream, s->n, q[0], q[1], q[2], q[3], tu,
                        tv, tw);
     LAUNCHED(s, "k_to_velocity<double>");
@@ -611,7 +610,8 @@ extern "C" int cns_get_state(cns_solver *s, double *r, double *u, double *v, dou
 // dilatation + fused RHS/update of one stage
 static int run_stage(cns_solver *s, int in_set, Params P, StagePtrs<double> A) {
     const bool wallx = !s->cfg.periodic_x, nonuni = s->cfg.nonuniform_x != 0;
-    for (int q = 0; q < 5; q++) A.qin[q] = s->sets[in_set][q];
+    A.qin = s->sets[in_set];
+    A.fs = s->n;
     A.dil = s->d_dil;
     fill_unused(A);
     HIPCHK(s, launch_dil<double>(s->SV, wallx, nonuni, s->nblocks, s->stream, P, A, s->d_dil));
@@ -643,19 +643,40 @@ extern "C" int cns_rhs(cns_solver *s, double *rr, double *ru, double *rv, double
     if (!s->have_state) return fail(s, CNS_ERR_STATE, "cns_rhs: no state set");
     HIPCHK(s, hipSetDevice(s->dev));
     Params P = s->base;
-    P.mode = MODE_RHS;
+    P.write_q = 0;
+    P.write_r = 1;
     P.want_visc_max = 1;
     StagePtrs<double> A;
     std::memset(&A, 0, sizeof(A));
-    for (int q = 0; q < 5; q++) A.rout[q] = s->rbuf[0][q];
+    A.rout = s->rbuf[0];
     int rc = run_stage(s, s->cur, P, A);
     if (rc) return rc;
     s->visc_from_state = false;
     double *dst[5] = {rr, ru, rv, rw, re};
     const size_t b = s->n * sizeof(double);
-    for (int q = 0; q < 5; q++) HIPCHK(s, hipMemcpyAsync(dst[q], s->rbuf[0][q], b, hipMemcpyDeviceToHost, s->stream));
+    for (int q = 0; q < 5; q++)
+        HIPCHK(s, hipMemcpyAsync(dst[q], s->rbuf[0] + q * s->n, b, hipMemcpyDeviceToHost, s->stream));
     HIPCHK(s, hipStreamSynchronize(s->stream));
     return CNS_OK;
+}
+
+// Launch parameters of one stage update
+// q_out = w_qc q_in + w_q0 q0 + w_acc acc + dt (w_ra ra + w_rb rb + w_r rhs).
+static Params stage_params(const cns_solver *s, double w_qc, double w_q0, double w_acc, double w_ra, double w_rb,
+                           double w_r) {
+    Params P = s->base;
+    P.write_q = 1;
+    P.w_qc = w_qc;
+    P.w_q0 = w_q0;
+    P.w_acc = w_acc;
+    P.w_ra = w_ra;
+    P.w_rb = w_rb;
+    P.w_r = w_r;
+    P.use_q0 = w_q0 != 0.0;
+    P.use_acc = w_acc != 0.0;
+    P.use_ra = w_ra != 0.0;
+    P.use_rb = w_rb != 0.0;
+    return P;
 }
 
 static const double SMR_ALPHA[3] = {0., -17. / 60., -5. / 12.};
@@ -703,65 +724,65 @@ static int queue_steps(cns_solver *s, int nsteps, bool want_par, bool want_time)
             LAUNCHED(s, "k_final");
         }
         const bool need_visc = ((istep + 1) % c.check_cfl == 0) || (istep == nsteps - 1);
-        Params P = s->base;
         int rc;
         if (c.scheme == CNS_RK3_LOWSTORAGE) {
+            // q <- q + dt (alpha_s r_prev + beta_s r_new)  (sumLowStorageRK3)
             const int a = s->cur, b = 1 - s->cur;
-            int in[3] = {a, b, a}, outs[3] = {b, a, b};
+            const int in[3] = {a, b, a}, outs[3] = {b, a, b};
             for (int st = 0; st < 3; st++) {
-                P.mode = MODE_SMR;
-                P.stage = st;
-                P.alpha = SMR_ALPHA[st];
-                P.beta = SMR_BETA[st];
+                Params P = stage_params(s, /*w_qc*/ 1, 0, 0, /*w_ra*/ st ? SMR_ALPHA[st] : 0, 0, SMR_BETA[st]);
+                P.write_r = st < 2;
                 P.want_visc_max = (st == 2 && need_visc) ? 1 : 0;
-                StagePtrs<double> A;
-                std::memset(&A, 0, sizeof(A));
-                for (int q = 0; q < 5; q++) {
-                    A.qout[q] = s->sets[outs[st]][q];
-                    if (st == 0) A.rout[q] = s->rbuf[0][q];
-                    if (st == 1) {
-                        A.ra[q] = s->rbuf[0][q];
-                        A.rout[q] = s->rbuf[1][q];
-                    }
-                    if (st == 2) A.ra[q] = s->rbuf[1][q];
-                }
+                StagePtrs<double> A{};
+                A.qout = s->sets[outs[st]];
+                A.rout = st < 2 ? s->rbuf[st] : nullptr;
+                A.ra = st == 0 ? nullptr : s->rbuf[st - 1];
                 if ((rc = run_stage(s, in[st], P, A))) return rc;
             }
             s->cur = b;
         } else if (c.scheme == CNS_RK3_KUTTA) {
+            // q1 = q0 + dt/2 k1; q2 = q0 + dt (2 k2 - k1); q = q0 + dt (k1 + 4 k2 + k3)/6
             const int q0 = s->cur, s1 = (s->cur + 1) % 3, s2 = (s->cur + 2) % 3;
-            int in[3] = {q0, s1, s2}, outs[3] = {s1, s2, q0};
+            const int in[3] = {q0, s1, s2}, outs[3] = {s1, s2, q0};
             for (int st = 0; st < 3; st++) {
-                P.mode = MODE_KUTTA;
-                P.stage = st;
+                Params P;
+                if (st == 0) P = stage_params(s, 1, 0, 0, 0, 0, 0.5);
+                else if (st == 1) P = stage_params(s, 0, 1, 0, -1.0, 0, 2.0);
+                else P = stage_params(s, 0, 1, 0, 1.0 / 6.0, 4.0 / 6.0, 1.0 / 6.0);
+                P.write_r = st < 2;
                 P.want_visc_max = (st == 2 && need_visc) ? 1 : 0;
-                StagePtrs<double> A;
-                std::memset(&A, 0, sizeof(A));
-                for (int q = 0; q < 5; q++) {
-                    A.qout[q] = s->sets[outs[st]][q];
-                    A.q0[q] = s->sets[q0][q];
-                    if (st < 2) A.rout[q] = s->rbuf[st][q];
-                    A.ra[q] = s->rbuf[0][q];
-                    A.rb[q] = s->rbuf[1][q];
-                }
+                StagePtrs<double> A{};
+                A.qout = s->sets[outs[st]];
+                A.q0 = s->sets[q0];
+                A.rout = st < 2 ? s->rbuf[st] : nullptr;
+                A.ra = s->rbuf[0];
+                A.rb = s->rbuf[1];
                 if ((rc = run_stage(s, in[st], P, A))) return rc;
             }
             s->cur = q0;
         } else {
-            // sets: A = current, B/D stage inputs, C accumulator
+            // classic RK4 with an accumulator set: acc = q0 + dt/6 k1 + dt/3 k2 + dt/3 k3,
+            // q = acc + dt/6 k4.  Sets: A = current, B/D stage inputs, C accumulator.
             const int sa = s->cur, sb = (s->cur + 1) % 4, scc = (s->cur + 2) % 4, sd = (s->cur + 3) % 4;
-            int in[4] = {sa, sb, sd, sb}, outs[4] = {sb, sd, sb, sa};
+            const int in[4] = {sa, sb, sd, sb}, outs[4] = {sb, sd, sb, sa};
             for (int st = 0; st < 4; st++) {
-                P.mode = MODE_RK4;
-                P.stage = st;
-                P.want_visc_max = (st == 3 && need_visc) ? 1 : 0;
-                StagePtrs<double> A;
-                std::memset(&A, 0, sizeof(A));
-                for (int q = 0; q < 5; q++) {
-                    A.qout[q] = s->sets[outs[st]][q];
-                    A.q0[q] = s->sets[sa][q];
-                    A.acc[q] = s->sets[scc][q];
+                Params P;
+                if (st == 0) P = stage_params(s, 1, 0, 0, 0, 0, 0.5);
+                else if (st == 1) P = stage_params(s, 0, 1, 0, 0, 0, 0.5);
+                else if (st == 2) P = stage_params(s, 0, 1, 0, 0, 0, 1.0);
+                else P = stage_params(s, 0, 0, 1, 0, 0, 1.0 / 6.0);
+                if (st < 3) {
+                    P.write_acc = 1;
+                    P.use_acc = st > 0;
+                    P.v_qc = st == 0 ? 1.0 : 0.0;
+                    P.v_acc = st == 0 ? 0.0 : 1.0;
+                    P.v_r = st == 0 ? 1.0 / 6.0 : 1.0 / 3.0;
                 }
+                P.want_visc_max = (st == 3 && need_visc) ? 1 : 0;
+                StagePtrs<double> A{};
+                A.qout = s->sets[outs[st]];
+                A.q0 = s->sets[sa];
+                A.acc = s->sets[scc];
                 if ((rc = run_stage(s, in[st], P, A))) return rc;
             }
             s->cur = sa;
@@ -838,7 +859,8 @@ extern "C" int cns_set_sponge(cns_solver *s, const double *sigma_x, const double
     HIPCHK(s, hipMemcpy(s->d_spz, sigma_z, mz * sizeof(double), hipMemcpyHostToDevice));
     const double *src[5] = {rref, ruref, rvref, rwref, reref};
     for (int q = 0; q < 5; q++)
-        HIPCHK(s, hipMemcpy(s->d_ref[q], src[q], (size_t)mx * mz * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(s, hipMemcpy(s->d_ref + (size_t)q * mx * mz, src[q], (size_t)mx * mz * sizeof(double),
+                            hipMemcpyHostToDevice));
     return CNS_OK;
 }
 

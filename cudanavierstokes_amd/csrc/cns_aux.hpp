@@ -82,9 +82,10 @@ __global__ void k_cfl(Params P, StagePtrs<R> A, int visc_from_state, Scalars *sc
     double cmax = 0.0, vmax = 0.0;
     for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x) {
         const int i = (int)(g % P.mx);
-        const double r = (double)A.qin[0][g];
-        const double u = (double)A.qin[1][g] / r, v = (double)A.qin[2][g] / r, w = (double)A.qin[3][g] / r;
-        const double ien = (double)A.qin[4][g] / r - 0.5 * (u * u + v * v + w * w);
+        const long fs = A.fs;
+        const double r = (double)A.qin[g];
+        const double u = (double)A.qin[g + fs] / r, v = (double)A.qin[g + 2 * fs] / r, w = (double)A.qin[g + 3 * fs] / r;
+        const double ien = (double)A.qin[g + 4 * fs] / r - 0.5 * (u * u + v * v + w * w);
         const double sos = sqrt(P.gam * (P.gam - 1) * ien);
         const double dx = P.dxv[i];
         const double c1 = (fabs(u) + sos) / dx, c2 = (fabs(v) + sos) * P.d_dy, c3 = (fabs(w) + sos) * P.d_dz;
@@ -133,12 +134,13 @@ __global__ void k_partials(Params P, StagePtrs<R> A, double *part) {
     for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x) {
         const int i = (int)(g % P.mx);
         const double wgt = P.dxv[i];
-        const double r = (double)A.qin[0][g];
-        const double ru = (double)A.qin[1][g], rv = (double)A.qin[2][g], rw = (double)A.qin[3][g];
+        const long fs = A.fs;
+        const double r = (double)A.qin[g];
+        const double ru = (double)A.qin[g + fs], rv = (double)A.qin[g + 2 * fs], rw = (double)A.qin[g + 3 * fs];
         const double u = ru / r, v = rv / r, w = rw / r;
         s[0] += r * wgt / P.d_dy / P.d_dz;
         s[1] += rw * wgt / P.d_dy / P.d_dz;
-        s[2] += (double)A.qin[4][g] * wgt / P.d_dy / P.d_dz;
+        s[2] += (double)A.qin[g + 4 * fs] * wgt / P.d_dy / P.d_dz;
         s[3] += (u * u + v * v + w * w) * wgt / P.d_dy / P.d_dz;
     }
     __shared__ double sh[NPART][RED_THREADS];

@@ -16,38 +16,43 @@
 
 namespace cns {
 
-enum { MODE_RHS = 0, MODE_SMR = 1, MODE_KUTTA = 2, MODE_RK4 = 3 };
-
 // Per-launch parameters (passed by value as a kernel argument).
 struct Params {
     int mx, my, mz;
     int zc;             // planes per z chunk
     int ntx, nty, nzc;  // tiles along x, y and chunks along z
     int forcing, sponge, fix_y_work, unit_visc;
-    int mode, stage;
     int want_visc_max;  // accumulate max(mu/dx^2, ...) for the next CFL check
     int xcd_remap;
+    // stage update, one linear combination per conserved field:
+    //   q_out = w_qc q_in + w_q0 q0 + w_acc acc + dt (w_ra ra + w_rb rb + w_r rhs)
+    //   acc   = v_qc q_in + v_acc acc + dt (v_r rhs)        (when write_acc)
+    //   r_out = rhs                                          (when write_r)
+    int use_q0, use_acc, use_ra, use_rb, write_q, write_r, write_acc;
+    double w_qc, w_q0, w_acc, w_ra, w_rb, w_r, v_qc, v_acc, v_r;
     double d_dx, d_dy, d_dz, d_d2x, d_d2y, d_d2z;
     double Rgas, gam, cvInv, Re, Pr, Ec, viscexp, invPrEc;
     double twall_bot, twall_top;
-    double alpha, beta;  // low-storage RK3 stage coefficients
     double cF[4], cS[5], cVF[4], cVS[5];
     const double *xp, *vsx, *dxv, *spx, *spz;
-    const double *ref[5];
+    const double *ref;  // 5 sponge reference fields of mx*mz, contiguous
     const double *dt, *dpdz;
     unsigned long long *visc_max_out;
 };
 
+// Field sets: the five conserved fields (rho, rho u, rho v, rho w, rho E) of
+// one state are one allocation, field q at offset q*fs.
 template <typename R>
 struct StagePtrs {
-    const R *qin[5];  // stage input (read with stencils)
-    const R *dil;     // dilatation of qin
-    R *qout[5];       // updated state
-    R *rout[5];       // right-hand side output
-    const R *ra[5];   // previous rhs (SMR r_prev; Kutta k1)
-    const R *rb[5];   // Kutta k2
-    const R *q0[5];   // step-start state (Kutta, RK4)
-    R *acc[5];        // RK4 accumulator
+    const R *qin;  // stage input (read with stencils)
+    const R *dil;  // dilatation of qin (one field)
+    R *qout;       // updated state
+    R *rout;       // right-hand side output
+    const R *ra;   // previous rhs (low-storage RK3 r_prev; Kutta k1)
+    const R *rb;   // Kutta k2
+    const R *q0;   // step-start state (Kutta, RK4)
+    R *acc;        // RK4 accumulator
+    long fs;       // field stride (points per field)
 };
 
 __device__ __forceinline__ int wrapi(int a, int n) {

@@ -43,9 +43,10 @@ __device__ __forceinline__ void load_prim(const Params &P, const StagePtrs<R> &A
         gi = wrapi(gi, P.mx);
     }
     const long g = gi + (long)P.mx * (gj + (long)P.my * k);
-    R r = A.qin[0][g];
+    const long fs = A.fs;
+    R r = A.qin[g];
     R u, v, w, t, h, mu;
-    eos<R>(P, r, A.qin[1][g], A.qin[2][g], A.qin[3][g], A.qin[4][g], u, v, w, t, h, mu);
+    eos<R>(P, r, A.qin[g + fs], A.qin[g + 2 * fs], A.qin[g + 3 * fs], A.qin[g + 4 * fs], u, v, w, t, h, mu);
     R d = A.dil[g];
     if (WALLX && side != 0) wall_ghost<R>(P, side < 0 ? R(P.twall_bot) : R(P.twall_top), r, u, v, w, t, h, mu);
     q[QR] = r;
@@ -290,6 +291,10 @@ __global__ __launch_bounds__(TX *TY) void k_rhs(Params P, StagePtrs<R> A) {
     }
 
     R win[NQ][W];
+#pragma unroll
+    for (int q = 0; q < NQ; q++)
+#pragma unroll
+        for (int a = 0; a < W; a++) win[q][a] = R(0);
     if (act) {
 #pragma unroll
         for (int a = 0; a < W - 1; a++) {
@@ -300,7 +305,7 @@ __global__ __launch_bounds__(TX *TY) void k_rhs(Params P, StagePtrs<R> A) {
         }
     }
     double vmax = 0.0;
-    const R dt = (P.mode != MODE_RHS) ? R(*P.dt) : R(0);
+    const R dt = P.write_q ? R(*P.dt) : R(0);
     const R dpdz = P.forcing ? R(*P.dpdz) : R(0);
 
     for (int k = z0; k < z1; ++k) {
@@ -361,14 +366,15 @@ __global__ __launch_bounds__(TX *TY) void k_rhs(Params P, StagePtrs<R> A) {
             rhs[4] = (rx[4] + ry[4]) + (rz[4] + dpdz * wc);
 
             const long gidx = i + (long)P.mx * (j + (long)P.my * k);
+            const long fs = A.fs;
             R qc[5];
 #pragma unroll
-            for (int q = 0; q < 5; q++) qc[q] = A.qin[q][gidx];
+            for (int q = 0; q < 5; q++) qc[q] = A.qin[gidx + q * fs];
             if (P.sponge) {
                 const R sg = R(P.spx[i] + P.spz[k]);
-                const long i2 = i + (long)P.mx * k;
+                const long i2 = i + (long)P.mx * k, rs = (long)P.mx * P.mz;
 #pragma unroll
-                for (int q = 0; q < 5; q++) rhs[q] += sg * (R(P.ref[q][i2]) - qc[q]);
+                for (int q = 0; q < 5; q++) rhs[q] += sg * (R(P.ref[i2 + q * rs]) - qc[q]);
             }
             if (P.want_visc_max) {
                 const double mu = (double)win[QM][H];
@@ -376,51 +382,24 @@ __global__ __launch_bounds__(TX *TY) void k_rhs(Params P, StagePtrs<R> A) {
                 const double vm = v1 >= (v2 >= v3 ? v2 : v3) ? v1 : (v2 >= v3 ? v2 : v3);
                 vmax = vmax >= vm ? vmax : vm;
             }
-            const int st = P.stage;
-            switch (P.mode) {
-            case MODE_RHS:
+            // stage update: straight-line per field, uniform flags only
 #pragma unroll
-                for (int q = 0; q < 5; q++) A.rout[q][gidx] = rhs[q];
-                break;
-            case MODE_SMR:
-#pragma unroll
-                for (int q = 0; q < 5; q++) {
-                    const R upd = (st == 0) ? R(P.beta) * rhs[q] : R(P.alpha) * A.ra[q][gidx] + R(P.beta) * rhs[q];
-                    A.qout[q][gidx] = qc[q] + dt * upd;
-                    if (st < 2) A.rout[q][gidx] = rhs[q];
+            for (int q = 0; q < 5; q++) {
+                const long gq = gidx + q * fs;
+                R base = R(P.w_qc) * qc[q];
+                if (P.use_q0) base += R(P.w_q0) * A.q0[gq];
+                R acc_old = R(0);
+                if (P.use_acc) {
+                    acc_old = A.acc[gq];
+                    base += R(P.w_acc) * acc_old;
                 }
-                break;
-            case MODE_KUTTA:
-#pragma unroll
-                for (int q = 0; q < 5; q++) {
-                    if (st == 0) {
-                        A.qout[q][gidx] = qc[q] + rhs[q] * dt / R(2.0);
-                        A.rout[q][gidx] = rhs[q];
-                    } else if (st == 1) {
-                        A.qout[q][gidx] = A.q0[q][gidx] + (2 * rhs[q] - A.ra[q][gidx]) * dt;
-                        A.rout[q][gidx] = rhs[q];
-                    } else {
-                        A.qout[q][gidx] = A.q0[q][gidx] + dt * (A.ra[q][gidx] + 4 * A.rb[q][gidx] + rhs[q]) / R(6.);
-                    }
-                }
-                break;
-            default: /* MODE_RK4 */
-#pragma unroll
-                for (int q = 0; q < 5; q++) {
-                    if (st == 0) {
-                        A.qout[q][gidx] = qc[q] + (R(0.5) * dt) * rhs[q];
-                        A.acc[q][gidx] = qc[q] + (dt / R(6.0)) * rhs[q];
-                    } else if (st == 1) {
-                        A.qout[q][gidx] = A.q0[q][gidx] + (R(0.5) * dt) * rhs[q];
-                        A.acc[q][gidx] = A.acc[q][gidx] + (dt / R(3.0)) * rhs[q];
-                    } else if (st == 2) {
-                        A.qout[q][gidx] = A.q0[q][gidx] + dt * rhs[q];
-                        A.acc[q][gidx] = A.acc[q][gidx] + (dt / R(3.0)) * rhs[q];
-                    } else {
-                        A.qout[q][gidx] = A.acc[q][gidx] + (dt / R(6.0)) * rhs[q];
-                    }
-                }
-                break;
+                R inc = R(0);
+                if (P.use_ra) inc += R(P.w_ra) * A.ra[gq];
+                if (P.use_rb) inc += R(P.w_rb) * A.rb[gq];
+                inc += R(P.w_r) * rhs[q];
+                if (P.write_q) A.qout[gq] = base + dt * inc;
+                if (P.write_r) A.rout[gq] = rhs[q];
+                if (P.write_acc) A.acc[gq] = R(P.v_qc) * qc[q] + R(P.v_acc) * acc_old + dt * (R(P.v_r) * rhs[q]);
             }
         }
 #pragma unroll
@@ -470,9 +449,9 @@ __global__ __launch_bounds__(TX *TY) void k_dil(Params P, StagePtrs<R> A, R *dil
             gi = wrapi(gi, P.mx);
         }
         const long g = gi + P.mx * (long)gj + mxy * k;
-        const R inv = R(1) / A.qin[0][g];
-        u = A.qin[1][g] * inv;
-        v = A.qin[2][g] * inv;
+        const R inv = R(1) / A.qin[g];
+        u = A.qin[g + A.fs] * inv;
+        v = A.qin[g + 2 * A.fs] * inv;
         if (ghost) {
             u = -u;
             v = -v;
@@ -480,7 +459,7 @@ __global__ __launch_bounds__(TX *TY) void k_dil(Params P, StagePtrs<R> A, R *dil
     };
     auto w_at = [&](int k) -> R {
         const long g = i + P.mx * (long)j + mxy * k;
-        return A.qin[3][g] / A.qin[0][g];
+        return A.qin[g + 3 * A.fs] / A.qin[g];
     };
 
     R ww[W];
