@@ -272,6 +272,8 @@ static void fill_params(cns_solver *s) {
     P.Ec = s->Ec;
     P.viscexp = c.visc_exp;
     P.invPrEc = 1.0 / (c.prandtl * s->Ec);
+    P.invRe = 1.0 / c.reynolds;
+    P.cpR = s->Rgas * c.gamma / (c.gamma - 1.0);
     P.twall_bot = c.twall_bot;
     P.twall_top = c.twall_top;
     for (int i = 0; i < 4; i++) {

@@ -31,7 +31,7 @@ struct Params {
     int use_q0, use_acc, use_ra, use_rb, write_q, write_r, write_acc;
     double w_qc, w_q0, w_acc, w_ra, w_rb, w_r, v_qc, v_acc, v_r;
     double d_dx, d_dy, d_dz, d_d2x, d_d2y, d_d2z;
-    double Rgas, gam, cvInv, Re, Pr, Ec, viscexp, invPrEc;
+    double Rgas, gam, cvInv, Re, Pr, Ec, viscexp, invPrEc, invRe, cpR;
     double twall_bot, twall_top;
     double cF[4], cS[5], cVF[4], cVS[5];
     const double *xp, *vsx, *dxv, *spx, *spz;
@@ -53,6 +53,74 @@ struct StagePtrs {
     const R *q0;   // step-start state (Kutta, RK4)
     R *acc;        // RK4 accumulator
     long fs;       // field stride (points per field)
+};
+
+// Central-difference weights of order 2s as compile-time constants (exact
+// rationals, one rounding each, so they equal the literal tables of
+// src/globals.h:69-95 bitwise):
+//   D1 f = sum_{k=1..s} a_k (f[i+k] - f[i-k]),  a_k = (-1)^{k+1} (s!)^2 / (k (s-k)! (s+k)!)
+//   D2 f = b_0 f[i] + sum_{k=1..s} b_k (f[i+k] + f[i-k]),
+//          b_k = 2 (-1)^{k+1} (s!)^2 / (k^2 (s-k)! (s+k)!),  b_0 = -2 sum b_k
+// The reference's coeffF[it] is -a_{s-it} and coeffS[it] is b_{s-it}.
+constexpr long long c_fact(int n) {
+    long long f = 1;
+    for (int i = 2; i <= n; i++) f *= i;
+    return f;
+}
+constexpr long long c_gcd(long long a, long long b) {
+    a = a < 0 ? -a : a;
+    b = b < 0 ? -b : b;
+    while (b) {
+        long long t = a % b;
+        a = b;
+        b = t;
+    }
+    return a ? a : 1;
+}
+constexpr double fd1(int s, int k) {
+    long long num = c_fact(s) * c_fact(s), den = (long long)k * c_fact(s - k) * c_fact(s + k);
+    const long long g = c_gcd(num, den);
+    num /= g;
+    den /= g;
+    return ((k % 2) ? (double)num : -(double)num) / (double)den;
+}
+constexpr double fd2(int s, int k) {
+    if (k > 0) {
+        long long num = 2 * c_fact(s) * c_fact(s), den = (long long)k * k * c_fact(s - k) * c_fact(s + k);
+        const long long g = c_gcd(num, den);
+        num /= g;
+        den /= g;
+        return ((k % 2) ? (double)num : -(double)num) / (double)den;
+    }
+    long long cn = 0, cd = 1;  // running sum of b_k as a reduced fraction
+    for (int j = 1; j <= s; j++) {
+        long long num = 2 * c_fact(s) * c_fact(s), den = (long long)j * j * c_fact(s - j) * c_fact(s + j);
+        long long g = c_gcd(num, den);
+        num /= g;
+        den /= g;
+        if (!(j % 2)) num = -num;
+        long long nn = cn * den + num * cd, dn = cd * den;
+        g = c_gcd(nn, dn);
+        cn = nn / g;
+        cd = dn / g;
+    }
+    long long nc = -2 * cn, dc = cd;
+    const long long g = c_gcd(nc, dc);
+    return (double)(nc / g) / (double)(dc / g);
+}
+
+// The weights of one order, evaluated in constant initialisers so that no
+// arithmetic of fd1/fd2 survives into device code.
+template <int S>
+struct FD {
+    static constexpr double A1 = fd1(S, 1), A2 = S >= 2 ? fd1(S, 2) : 0.0, A3 = S >= 3 ? fd1(S, 3) : 0.0,
+                            A4 = S >= 4 ? fd1(S, 4) : 0.0;
+    static constexpr double B0 = fd2(S, 0), B1 = fd2(S, 1), B2 = S >= 2 ? fd2(S, 2) : 0.0,
+                            B3 = S >= 3 ? fd2(S, 3) : 0.0, B4 = S >= 4 ? fd2(S, 4) : 0.0;
+    __host__ __device__ static constexpr double a(int k) { return k == 1 ? A1 : k == 2 ? A2 : k == 3 ? A3 : A4; }
+    __host__ __device__ static constexpr double b(int k) {
+        return k == 0 ? B0 : k == 1 ? B1 : k == 2 ? B2 : k == 3 ? B3 : B4;
+    }
 };
 
 __device__ __forceinline__ int wrapi(int a, int n) {
@@ -80,12 +148,13 @@ __device__ __forceinline__ void eos(const Params &P, R r, R ru, R rv, R rw, R re
     u = ru * inv;
     v = rv * inv;
     w = rw * inv;
-    R en = re * inv - R(0.5) * (u * u + v * v + w * w);
+    const R k2 = R(0.5) * (u * u + v * v + w * w);
+    R en = re * inv - k2;
     t = R(P.cvInv) * en;
-    R p = r * R(P.Rgas) * t;
-    h = (re + p) * inv;
+    // total enthalpy (rhoE + p)/rho = cp T + |u|^2/2 with cp = gam R/(gam-1)
+    h = t * R(P.cpR) + k2;
     R suth = P.unit_visc ? t : R(pow((double)t, P.viscexp));
-    mu = suth / R(P.Re);
+    mu = suth * R(P.invRe);
 }
 
 // Isothermal no-slip wall ghost from its mirror point (src/boundary.h:79-148):
@@ -99,12 +168,15 @@ __device__ __forceinline__ void wall_ghost(const Params &P, R tw, R &r, R &u, R 
     w = -w;
     t = R(2.0) * tw - t;
     R suth = P.unit_visc ? t : R(pow((double)t, P.viscexp));
-    mu = suth / R(P.Re);
+    mu = suth * R(P.invRe);
     h = t * R(P.Rgas) * R(P.gam) / (R(P.gam) - R(1.0)) + R(0.5) * (u * u + v * v + w * w);
     r = p / (R(P.Rgas) * t);
 }
 
 // Quantity indices of the staged arrays.
 enum { QR = 0, QU, QV, QW, QH, QT, QM, QD, NQ };
+// Quantities of the z register window: h is rebuilt from T and u (h = cp T +
+// |u|^2/2) and, for mu = T/Re, mu from T, so the window holds 6 or 7 of them.
+enum { ZR = 0, ZU, ZV, ZW, ZT, ZD, ZM };
 
 }  // namespace cns

@@ -10,21 +10,27 @@
 
 namespace cns {
 
-template <typename R, int S, int SV>
-static hipError_t go_rhs(bool wallx, bool nonuni, int nb, hipStream_t st, const Params &P, const StagePtrs<R> &A) {
+template <typename R, int S, int SV, bool UNITV>
+static hipError_t go_rhs_v(bool wallx, bool nonuni, int nb, hipStream_t st, const Params &P, const StagePtrs<R> &A) {
     const dim3 g(nb), b(TILE_X * TILE_Y);
     if (wallx) {
         if (nonuni)
-            hipLaunchKernelGGL((k_rhs<R, S, SV, TILE_X, TILE_Y, true, true>), g, b, 0, st, P, A);
+            hipLaunchKernelGGL((k_rhs<R, S, SV, TILE_X, TILE_Y, true, true, UNITV>), g, b, 0, st, P, A);
         else
-            hipLaunchKernelGGL((k_rhs<R, S, SV, TILE_X, TILE_Y, true, false>), g, b, 0, st, P, A);
+            hipLaunchKernelGGL((k_rhs<R, S, SV, TILE_X, TILE_Y, true, false, UNITV>), g, b, 0, st, P, A);
     } else {
         if (nonuni)
-            hipLaunchKernelGGL((k_rhs<R, S, SV, TILE_X, TILE_Y, false, true>), g, b, 0, st, P, A);
+            hipLaunchKernelGGL((k_rhs<R, S, SV, TILE_X, TILE_Y, false, true, UNITV>), g, b, 0, st, P, A);
         else
-            hipLaunchKernelGGL((k_rhs<R, S, SV, TILE_X, TILE_Y, false, false>), g, b, 0, st, P, A);
+            hipLaunchKernelGGL((k_rhs<R, S, SV, TILE_X, TILE_Y, false, false, UNITV>), g, b, 0, st, P, A);
     }
     return hipGetLastError();
+}
+
+template <typename R, int S, int SV>
+static hipError_t go_rhs(bool wallx, bool nonuni, int nb, hipStream_t st, const Params &P, const StagePtrs<R> &A) {
+    return P.unit_visc ? go_rhs_v<R, S, SV, true>(wallx, nonuni, nb, st, P, A)
+                       : go_rhs_v<R, S, SV, false>(wallx, nonuni, nb, st, P, A);
 }
 
 template <typename R, int SV>

@@ -60,7 +60,8 @@ __device__ __forceinline__ void load_prim(const Params &P, const StagePtrs<R> &A
 }
 
 // Viscous-order first and second derivatives of u, v, w along DIR
-// (src/cuda_derivs.h:192-224 for x, 248-268 for y, 292-312 for z).
+// (src/cuda_derivs.h:192-224 for x, 248-268 for y, 292-312 for z).  The
+// weights are compile-time constants; the spacing multiplies each sum once.
 template <typename R, int SV, int DIR, bool NONUNI, class Acc>
 __device__ __forceinline__ void vel_stencils(const Params &P, Acc f, const double *vs, double xpi, R d1[3],
                                              R d2[3]) {
@@ -69,30 +70,24 @@ __device__ __forceinline__ void vel_stencils(const Params &P, Acc f, const doubl
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         const int q = QU + c;
-        R a1 = R(0);
+        R a1 = R(0), a2 = R(0);
 #pragma unroll
-        for (int it = 0; it < SV; it++) {
-            if (DIR == 0)
-                a1 += R(P.cVF[it]) * (f(q, it - SV) - f(q, SV - it));
-            else
-                a1 += R(P.cVF[it]) * (f(q, it - SV) - f(q, SV - it)) * d;
+        for (int k = SV; k >= 1; k--) {
+            const R fp = f(q, k), fm = f(q, -k);
+            a1 += R(FD<SV>::a(k)) * (fp - fm);
+            a2 += R(FD<SV>::b(k)) * (fp + fm);
         }
-        if (DIR == 0) {
-            a1 = a1 * d;
-            if (NONUNI) a1 = a1 * R(xpi);
-        }
+        a1 = a1 * d;
+        if (DIR == 0 && NONUNI) a1 = a1 * R(xpi);
         d1[c] = a1;
-        R a2;
         if (DIR == 0 && NONUNI) {
-            a2 = R(0);
+            R b = R(0);
 #pragma unroll
-            for (int it = 0; it < 2 * SV + 1; it++) a2 += R(vs[it]) * f(q, it - SV);
+            for (int it = 0; it < 2 * SV + 1; it++) b += R(vs[it]) * f(q, it - SV);
+            d2[c] = b;
         } else {
-            a2 = R(P.cVS[SV]) * f(q, 0) * dd;
-#pragma unroll
-            for (int it = 0; it < SV; it++) a2 += R(P.cVS[it]) * (f(q, it - SV) + f(q, SV - it)) * dd;
+            d2[c] = (a2 + R(FD<SV>::b(0)) * f(q, 0)) * dd;
         }
-        d2[c] = a2;
     }
 }
 
@@ -108,86 +103,53 @@ __device__ __forceinline__ void dir_terms(const Params &P, Acc f, const double *
     const R lamc = muc * R(P.invPrEc);
     const R Rg = R(P.Rgas);
     // viscous-order first derivatives of mu, T, dil; second derivative of T
-    R d1m = R(0), d1t = R(0), d1d = R(0), d2t;
+    R d1m = R(0), d1t = R(0), d1d = R(0), d2t = R(0);
 #pragma unroll
-    for (int it = 0; it < SV; it++) {
-        const R c = R(P.cVF[it]);
-        if (DIR == 0) {
-            d1m += c * (f(QM, it - SV) - f(QM, SV - it));
-            d1t += c * (f(QT, it - SV) - f(QT, SV - it));
-            d1d += c * (f(QD, it - SV) - f(QD, SV - it));
-        } else {
-            d1m += c * (f(QM, it - SV) - f(QM, SV - it)) * d;
-            d1t += c * (f(QT, it - SV) - f(QT, SV - it)) * d;
-            d1d += c * (f(QD, it - SV) - f(QD, SV - it)) * d;
-        }
+    for (int k = SV; k >= 1; k--) {
+        const R a = R(FD<SV>::a(k));
+        const R tp = f(QT, k), tm = f(QT, -k);
+        d1m += a * (f(QM, k) - f(QM, -k));
+        d1t += a * (tp - tm);
+        d1d += a * (f(QD, k) - f(QD, -k));
+        d2t += R(FD<SV>::b(k)) * (tp + tm);
     }
-    if (DIR == 0) {
-        d1m = d1m * d;
-        d1t = d1t * d;
-        d1d = d1d * d;
-        if (NONUNI) {
-            d1m = d1m * R(xpi);
-            d1t = d1t * R(xpi);
-            d1d = d1d * R(xpi);
-        }
-    }
+    R xs = d;
+    if (DIR == 0 && NONUNI) xs = d * R(xpi);
+    d1m = d1m * xs;
+    d1t = d1t * xs;
+    d1d = d1d * xs;
     if (DIR == 0 && NONUNI) {
         d2t = R(0);
 #pragma unroll
         for (int it = 0; it < 2 * SV + 1; it++) d2t += R(vs[it]) * f(QT, it - SV);
     } else {
-        d2t = R(P.cVS[SV]) * f(QT, 0) * dd;
-#pragma unroll
-        for (int it = 0; it < SV; it++) d2t += R(P.cVS[it]) * (f(QT, it - SV) + f(QT, SV - it)) * dd;
+        d2t = (d2t + R(FD<SV>::b(0)) * f(QT, 0)) * dd;
     }
     // advective-order pressure gradient, p = rho R T
     R d1p = R(0);
 #pragma unroll
-    for (int it = 0; it < S; it++) {
-        const R pm = f(QR, it - S) * Rg * f(QT, it - S);
-        const R pp = f(QR, S - it) * Rg * f(QT, S - it);
-        if (DIR == 0)
-            d1p += R(P.cF[it]) * (pm - pp);
-        else
-            d1p += R(P.cF[it]) * (pm - pp) * d;
-    }
-    if (DIR == 0) {
-        d1p = d1p * d;
-        if (NONUNI) d1p = d1p * R(xpi);
-    }
+    for (int k = S; k >= 1; k--) d1p += R(FD<S>::a(k)) * (f(QR, k) * f(QT, k) - f(QR, -k) * f(QT, -k));
+    d1p = d1p * (Rg * xs);
     const R d1l = d1m * R(P.invPrEc);
 
-    // split-form convective fluxes, telescoped pair form
+    // split-form convective fluxes, telescoped pair form:
+    // -d(fgh)/dx ~ -(d/4) sum_l a_l [(f_i+f_{i+l})(g_i+g_{i+l})(h_i+h_{i+l}) - (f_{i-l}+f_i)(...)(...)]
     const int QA = (DIR == 0) ? QU : (DIR == 1 ? QV : QW);
     R fr = R(0), fu = R(0), fv = R(0), fw = R(0), fe = R(0);
     const R rc = f(QR, 0), ac = f(QA, 0), hc = f(QH, 0);
 #pragma unroll
-    for (int l = 1; l <= S; l++) {
-        const R c = R(P.cF[S - l]);
-        // pair (i, i+l)
-        R rs = rc + f(QR, l), as = ac + f(QA, l);
-        R qp = c * rs * as;
-        R pu = qp * (uc + f(QU, l)), pv = qp * (vc + f(QV, l)), pw = qp * (wc + f(QW, l));
-        R pe = qp * (hc + f(QH, l));
-        // pair (i-l, i)
-        R rs2 = f(QR, -l) + rc, as2 = f(QA, -l) + ac;
-        R qm = c * rs2 * as2;
+    for (int l = S; l >= 1; l--) {
+        const R c = R(FD<S>::a(l));
+        const R qp = c * ((rc + f(QR, l)) * (ac + f(QA, l)));
+        const R qm = c * ((f(QR, -l) + rc) * (f(QA, -l) + ac));
         fr += qp - qm;
-        fu += pu - qm * (f(QU, -l) + uc);
-        fv += pv - qm * (f(QV, -l) + vc);
-        fw += pw - qm * (f(QW, -l) + wc);
-        fe += pe - qm * (f(QH, -l) + hc);
+        fu += qp * (uc + f(QU, l)) - qm * (f(QU, -l) + uc);
+        fv += qp * (vc + f(QV, l)) - qm * (f(QV, -l) + vc);
+        fw += qp * (wc + f(QW, l)) - qm * (f(QW, -l) + wc);
+        fe += qp * (hc + f(QH, l)) - qm * (f(QH, -l) + hc);
     }
-    R sq = R(0.5) * d, sc = R(0.25) * d;
-    R cr = sq * fr, cu = sc * fu, cv = sc * fv, cw = sc * fw, ce = sc * fe;
-    if (DIR == 0 && NONUNI) {
-        cr = cr * R(xpi);
-        cu = cu * R(xpi);
-        cv = cv * R(xpi);
-        cw = cw * R(xpi);
-        ce = ce * R(xpi);
-    }
+    const R sq = R(-0.5) * xs, sc = R(-0.25) * xs;
+    const R cr = sq * fr, cu = sc * fu, cv = sc * fv, cw = sc * fw, ce = sc * fe;
 
     // gradient index map: 0 dudx 1 dvdx 2 dwdx 3 dudy 4 dvdy 5 dwdy 6 dudz 7 dvdz 8 dwdz
     R su, sv, sw, e;
@@ -217,7 +179,7 @@ __device__ __forceinline__ void dir_terms(const Params &P, Acc f, const double *
     sv = sv + d2v[1] * muc;
     sw = sw + d2v[2] * muc;
     e = e + uc * su + vc * sv + wc * sw;
-    const R ac3 = muc * d1d / R(3.0);
+    const R ac3 = muc * d1d * R(1.0 / 3.0);
     if (DIR == 0) {
         e = e + d2t * lamc;
         e = e + d1t * d1l;
@@ -257,12 +219,39 @@ __device__ __forceinline__ void block_max_atomic(double v, unsigned long long *o
     }
 }
 
-template <typename R, int S, int SV, int TX, int TY, bool WALLX, bool NONUNI>
+template <typename R, int S, int SV, int TX, int TY, bool WALLX, bool NONUNI, bool UNITV>
 __global__ __launch_bounds__(TX *TY) void k_rhs(Params P, StagePtrs<R> A) {
     constexpr int H = S, LX = TX + 2 * H, LY = TY + 2 * H, NT = TX * TY, W = 2 * H + 1;
+    constexpr int NZ = UNITV ? 6 : 7;
     constexpr int NH = 2 * H * TY + 2 * H * TX;
     __shared__ R sq[NQ][LY * LX];
     __shared__ double red[NT / 64 + 1];
+    // Stage-update weights, flags and output pointers live in LDS and are read
+    // only by the update, so they do not occupy scalar registers across the
+    // z march (the barriers in the loop keep these loads in place).
+    __shared__ double s_w[9];
+    __shared__ unsigned long long s_ptr[6];
+    __shared__ int s_flags;
+    if (threadIdx.x == 0) {
+        s_w[0] = P.w_qc;
+        s_w[1] = P.w_q0;
+        s_w[2] = P.w_acc;
+        s_w[3] = P.w_ra;
+        s_w[4] = P.w_rb;
+        s_w[5] = P.w_r;
+        s_w[6] = P.v_qc;
+        s_w[7] = P.v_acc;
+        s_w[8] = P.v_r;
+        s_ptr[0] = (unsigned long long)A.qout;
+        s_ptr[1] = (unsigned long long)A.rout;
+        s_ptr[2] = (unsigned long long)A.ra;
+        s_ptr[3] = (unsigned long long)A.rb;
+        s_ptr[4] = (unsigned long long)A.q0;
+        s_ptr[5] = (unsigned long long)A.acc;
+        s_flags = (P.use_q0 ? 1 : 0) | (P.use_acc ? 2 : 0) | (P.use_ra ? 4 : 0) | (P.use_rb ? 8 : 0) |
+                  (P.write_q ? 16 : 0) | (P.write_r ? 32 : 0) | (P.write_acc ? 64 : 0) | (P.sponge ? 128 : 0) |
+                  (P.want_visc_max ? 256 : 0);
+    }
 
     const int bb = remap_block(blockIdx.x, gridDim.x, P.xcd_remap);
     const int tile_x = bb % P.ntx;
@@ -290,18 +279,27 @@ __global__ __launch_bounds__(TX *TY) void k_rhs(Params P, StagePtrs<R> A) {
         }
     }
 
-    R win[NQ][W];
+    R win[NZ][W];
 #pragma unroll
-    for (int q = 0; q < NQ; q++)
+    for (int q = 0; q < NZ; q++)
 #pragma unroll
         for (int a = 0; a < W; a++) win[q][a] = R(0);
+    auto push = [&](const R *t, int a) {
+        win[ZR][a] = t[QR];
+        win[ZU][a] = t[QU];
+        win[ZV][a] = t[QV];
+        win[ZW][a] = t[QW];
+        win[ZT][a] = t[QT];
+        win[ZD][a] = t[QD];
+        if (!UNITV) win[NZ - 1][a] = t[QM];
+    };
+    const R cpR = R(P.cpR), invRe = R(P.invRe);
     if (act) {
 #pragma unroll
         for (int a = 0; a < W - 1; a++) {
             R t[NQ];
             load_prim<R, WALLX>(P, A, i, j, wrapi(z0 - H + a, P.mz), t);
-#pragma unroll
-            for (int q = 0; q < NQ; q++) win[q][a] = t[q];
+            push(t, a);
         }
     }
     double vmax = 0.0;
@@ -312,13 +310,28 @@ __global__ __launch_bounds__(TX *TY) void k_rhs(Params P, StagePtrs<R> A) {
         if (act) {
             R t[NQ];
             load_prim<R, WALLX>(P, A, i, j, wrapi(k + H, P.mz), t);
-#pragma unroll
-            for (int q = 0; q < NQ; q++) win[q][W - 1] = t[q];
+            push(t, W - 1);
         }
+        // z-window accessor; h and (for mu = T/Re) mu are rebuilt on the fly
+        auto FZ = [&](int q, int a) -> R {
+            switch (q) {
+            case QR: return win[ZR][H + a];
+            case QU: return win[ZU][H + a];
+            case QV: return win[ZV][H + a];
+            case QW: return win[ZW][H + a];
+            case QT: return win[ZT][H + a];
+            case QD: return win[ZD][H + a];
+            case QM: return UNITV ? win[ZT][H + a] * invRe : win[NZ - 1][H + a];
+            default: {
+                const R u = win[ZU][H + a], v = win[ZV][H + a], w = win[ZW][H + a];
+                return win[ZT][H + a] * cpR + R(0.5) * (u * u + v * v + w * w);
+            }
+            }
+        };
         __syncthreads();
         if (act) {
 #pragma unroll
-            for (int q = 0; q < NQ; q++) sq[q][cc] = win[q][H];
+            for (int q = 0; q < NQ; q++) sq[q][cc] = FZ(q, 0);
         } else {
             R t[NQ];
             load_prim<R, WALLX>(P, A, i, j, k, t);
@@ -347,63 +360,81 @@ __global__ __launch_bounds__(TX *TY) void k_rhs(Params P, StagePtrs<R> A) {
         if (act) {
             auto FX = [&](int q, int a) -> R { return sq[q][cc + a]; };
             auto FY = [&](int q, int a) -> R { return sq[q][cc + a * LX]; };
-            auto FZ = [&](int q, int a) -> R { return win[q][H + a]; };
+            // scheduling barriers between the phases keep the compiler from
+            // hoisting every LDS read of the point to the top (register pressure)
             R d1x[3], d2x[3], d1y[3], d2y[3], d1z[3], d2z[3];
             vel_stencils<R, SV, 0, NONUNI>(P, FX, vs, xpi, d1x, d2x);
+            __builtin_amdgcn_sched_barrier(0);
             vel_stencils<R, SV, 1, false>(P, FY, vs, xpi, d1y, d2y);
+            __builtin_amdgcn_sched_barrier(0);
             vel_stencils<R, SV, 2, false>(P, FZ, vs, xpi, d1z, d2z);
+            __builtin_amdgcn_sched_barrier(0);
             const R g[9] = {d1x[0], d1x[1], d1x[2], d1y[0], d1y[1], d1y[2], d1z[0], d1z[1], d1z[2]};
-            R rx[5], ry[5], rz[5];
-            dir_terms<R, S, SV, 0, NONUNI>(P, FX, vs, xpi, g, d2x, rx);
-            dir_terms<R, S, SV, 1, false>(P, FY, vs, xpi, g, d2y, ry);
-            dir_terms<R, S, SV, 2, false>(P, FZ, vs, xpi, g, d2z, rz);
-            const R wc = win[QW][H];
-            R rhs[5];
-            rhs[0] = (rx[0] + ry[0]) + rz[0];
-            rhs[1] = (rx[1] + ry[1]) + rz[1];
-            rhs[2] = (rx[2] + ry[2]) + rz[2];
-            rhs[3] = (rx[3] + ry[3]) + (rz[3] + dpdz);
-            rhs[4] = (rx[4] + ry[4]) + (rz[4] + dpdz * wc);
+            // x part (=), y part (+=), z part (+=) with the forcing, as
+            // src/cuda_rhs.cu:123-127,255-259,389-393 accumulate them
+            R rhs[5], part[5];
+            dir_terms<R, S, SV, 0, NONUNI>(P, FX, vs, xpi, g, d2x, rhs);
+            __builtin_amdgcn_sched_barrier(0);
+            dir_terms<R, S, SV, 1, false>(P, FY, vs, xpi, g, d2y, part);
+#pragma unroll
+            for (int q = 0; q < 5; q++) rhs[q] = rhs[q] + part[q];
+            __builtin_amdgcn_sched_barrier(0);
+            dir_terms<R, S, SV, 2, false>(P, FZ, vs, xpi, g, d2z, part);
+            __builtin_amdgcn_sched_barrier(0);
+            const R wc = win[ZW][H];
+            rhs[0] = rhs[0] + part[0];
+            rhs[1] = rhs[1] + part[1];
+            rhs[2] = rhs[2] + part[2];
+            rhs[3] = rhs[3] + (part[3] + dpdz);
+            rhs[4] = rhs[4] + (part[4] + dpdz * wc);
 
             const long gidx = i + (long)P.mx * (j + (long)P.my * k);
             const long fs = A.fs;
+            const int fl = __builtin_amdgcn_readfirstlane(s_flags);
             R qc[5];
 #pragma unroll
             for (int q = 0; q < 5; q++) qc[q] = A.qin[gidx + q * fs];
-            if (P.sponge) {
+            if (fl & 128) {
                 const R sg = R(P.spx[i] + P.spz[k]);
                 const long i2 = i + (long)P.mx * k, rs = (long)P.mx * P.mz;
 #pragma unroll
                 for (int q = 0; q < 5; q++) rhs[q] += sg * (R(P.ref[i2 + q * rs]) - qc[q]);
             }
-            if (P.want_visc_max) {
-                const double mu = (double)win[QM][H];
+            if (fl & 256) {
+                const double mu = (double)FZ(QM, 0);
                 const double v1 = mu / (dxvi * dxvi), v2 = mu * P.d_d2y, v3 = mu * P.d_d2z;
                 const double vm = v1 >= (v2 >= v3 ? v2 : v3) ? v1 : (v2 >= v3 ? v2 : v3);
                 vmax = vmax >= vm ? vmax : vm;
             }
-            // stage update: straight-line per field, uniform flags only
+            // stage update, one linear combination per field:
+            //   q_out = w_qc q_in + w_q0 q0 + w_acc acc + dt (w_ra ra + w_rb rb + w_r rhs)
+            R *const qout = (R *)s_ptr[0];
+            R *const rout = (R *)s_ptr[1];
+            const R *const ra = (const R *)s_ptr[2];
+            const R *const rb = (const R *)s_ptr[3];
+            const R *const q0 = (const R *)s_ptr[4];
+            R *const acc = (R *)s_ptr[5];
 #pragma unroll
             for (int q = 0; q < 5; q++) {
                 const long gq = gidx + q * fs;
-                R base = R(P.w_qc) * qc[q];
-                if (P.use_q0) base += R(P.w_q0) * A.q0[gq];
+                R base = R(s_w[0]) * qc[q];
+                if (fl & 1) base += R(s_w[1]) * q0[gq];
                 R acc_old = R(0);
-                if (P.use_acc) {
-                    acc_old = A.acc[gq];
-                    base += R(P.w_acc) * acc_old;
+                if (fl & 2) {
+                    acc_old = acc[gq];
+                    base += R(s_w[2]) * acc_old;
                 }
                 R inc = R(0);
-                if (P.use_ra) inc += R(P.w_ra) * A.ra[gq];
-                if (P.use_rb) inc += R(P.w_rb) * A.rb[gq];
-                inc += R(P.w_r) * rhs[q];
-                if (P.write_q) A.qout[gq] = base + dt * inc;
-                if (P.write_r) A.rout[gq] = rhs[q];
-                if (P.write_acc) A.acc[gq] = R(P.v_qc) * qc[q] + R(P.v_acc) * acc_old + dt * (R(P.v_r) * rhs[q]);
+                if (fl & 4) inc += R(s_w[3]) * ra[gq];
+                if (fl & 8) inc += R(s_w[4]) * rb[gq];
+                inc += R(s_w[5]) * rhs[q];
+                if (fl & 16) qout[gq] = base + dt * inc;
+                if (fl & 32) rout[gq] = rhs[q];
+                if (fl & 64) acc[gq] = R(s_w[6]) * qc[q] + R(s_w[7]) * acc_old + dt * (R(s_w[8]) * rhs[q]);
             }
         }
 #pragma unroll
-        for (int q = 0; q < NQ; q++)
+        for (int q = 0; q < NZ; q++)
 #pragma unroll
             for (int a = 0; a < W - 1; a++) win[q][a] = win[q][a + 1];
     }
@@ -497,13 +528,16 @@ __global__ __launch_bounds__(TX *TY) void k_dil(Params P, StagePtrs<R> A, R *dil
         if (act) {
             R a1 = R(0), a2 = R(0), a3 = R(0);
 #pragma unroll
-            for (int it = 0; it < SV; it++) {
-                a1 += R(P.cVF[it]) * (su[cc + it - SV] - su[cc + SV - it]);
-                a2 += R(P.cVF[it]) * (sv[cc + (it - SV) * LX] - sv[cc + (SV - it) * LX]) * R(P.d_dy);
-                a3 += R(P.cVF[it]) * (ww[H + it - SV] - ww[H + SV - it]) * R(P.d_dz);
+            for (int k = SV; k >= 1; k--) {
+                const R c = R(FD<SV>::a(k));
+                a1 += c * (su[cc + k] - su[cc - k]);
+                a2 += c * (sv[cc + k * LX] - sv[cc - k * LX]);
+                a3 += c * (ww[H + k] - ww[H - k]);
             }
             a1 = a1 * R(P.d_dx);
             if (NONUNI) a1 = a1 * R(xpi);
+            a2 = a2 * R(P.d_dy);
+            a3 = a3 * R(P.d_dz);
             dil[i + P.mx * (long)j + mxy * k] = a1 + a2 + a3;
         }
 #pragma unroll

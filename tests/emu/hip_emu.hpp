@@ -1,8 +1,9 @@
 // hip_emu.hpp -- CPU emulation of the HIP subset used by libcns (test
-// infrastructure).  Workgroups run one after another; the threads of a
-// workgroup run as std::threads with a std::barrier for __syncthreads, so
-// __shared__ can be a function-local static.  Device memory is host memory,
-// which lets AddressSanitizer check every kernel access on the CPU.
+// infrastructure).  Workgroups run one after another; the lanes of a
+// workgroup run as fibers on one host thread (or, with EMU_THREADS, as
+// std::threads with a std::barrier), so __shared__ can be a function-local
+// static.  Device memory is host memory, which lets AddressSanitizer check
+// every kernel access on the CPU.
 #pragma once
 #include <algorithm>
 #include <atomic>
@@ -14,6 +15,7 @@
 #include <functional>
 #include <thread>
 #include <vector>
+#include <ucontext.h>
 
 #define __global__
 #define __device__
@@ -34,7 +36,11 @@ inline thread_local emu_idx threadIdx, blockIdx, blockDim, gridDim;
 inline thread_local std::barrier<> *emu_bar = nullptr;
 inline double emu_shfl[1024];
 
-inline void __syncthreads() { emu_bar->arrive_and_wait(); }
+inline void __syncthreads();
+
+// scheduling hints and wave-uniform reads have no effect on the CPU
+#define __builtin_amdgcn_sched_barrier(m) ((void)(m))
+inline int __builtin_amdgcn_readfirstlane(int v) { return v; }
 
 inline double __shfl_xor(double v, int off) {
     const unsigned t = threadIdx.x;
@@ -113,8 +119,9 @@ inline hipError_t hipEventElapsedTime(float *f, hipEvent_t, hipEvent_t) {
 }
 inline hipError_t hipEventDestroy(hipEvent_t) { return hipSuccess; }
 
+#ifdef EMU_THREADS
 // Persistent worker pool: one host thread per workgroup lane, reused for
-// every workgroup of every launch.
+// every workgroup of every launch (used by the AddressSanitizer build).
 struct EmuPool {
     std::vector<std::thread> th;
     unsigned n = 0;
@@ -158,6 +165,7 @@ struct EmuPool {
     ~EmuPool() { shutdown(); }
 };
 inline EmuPool emu_pool;
+inline void __syncthreads() { emu_bar->arrive_and_wait(); }
 
 template <class F, class... Args>
 void emu_launch(F f, dim3 grid, dim3 block, Args... args) {
@@ -185,4 +193,91 @@ void emu_launch(F f, dim3 grid, dim3 block, Args... args) {
                 });
             }
 }
+#else
+// Fibers: every lane of a workgroup is a ucontext on one host thread;
+// __syncthreads switches back to the scheduler, which resumes the lanes in
+// order once all of them have arrived.  Lanes that return early leave the
+// barrier count, as waves that end do on the hardware.
+struct EmuFibers {
+    std::vector<ucontext_t> ctx;
+    std::vector<char *> stacks;
+    std::vector<int> state;  // 0 runnable, 1 at barrier, 2 finished
+    ucontext_t sched;
+    unsigned cur = 0, nt = 0;
+    std::function<void(unsigned)> task;
+    static constexpr size_t STACK = 256 * 1024;
+    void ensure(unsigned n) {
+        while (stacks.size() < n) stacks.push_back((char *)std::malloc(STACK));
+        if (ctx.size() < n) ctx.resize(n);
+        state.assign(n, 0);
+    }
+    ~EmuFibers() {
+        for (char *p : stacks) std::free(p);
+    }
+};
+inline EmuFibers emu_fib;
+inline dim3 emu_grid, emu_block;
+inline unsigned emu_bx, emu_by, emu_bz;
+
+inline void emu_set_ids(unsigned t) {
+    threadIdx.x = t % emu_block.x;
+    threadIdx.y = (t / emu_block.x) % emu_block.y;
+    threadIdx.z = t / (emu_block.x * emu_block.y);
+    blockIdx.x = emu_bx;
+    blockIdx.y = emu_by;
+    blockIdx.z = emu_bz;
+    blockDim.x = emu_block.x;
+    blockDim.y = emu_block.y;
+    blockDim.z = emu_block.z;
+    gridDim.x = emu_grid.x;
+    gridDim.y = emu_grid.y;
+    gridDim.z = emu_grid.z;
+}
+inline void emu_fiber_entry() {
+    const unsigned t = emu_fib.cur;
+    emu_fib.task(t);
+    emu_fib.state[t] = 2;
+    swapcontext(&emu_fib.ctx[t], &emu_fib.sched);
+}
+inline void __syncthreads() {
+    const unsigned t = emu_fib.cur;
+    emu_fib.state[t] = 1;
+    swapcontext(&emu_fib.ctx[t], &emu_fib.sched);
+}
+
+template <class F, class... Args>
+void emu_launch(F f, dim3 grid, dim3 block, Args... args) {
+    const unsigned nt = block.x * block.y * block.z;
+    emu_grid = grid;
+    emu_block = block;
+    emu_fib.ensure(nt);
+    emu_fib.nt = nt;
+    emu_fib.task = [&](unsigned t) { f(args...); };
+    for (emu_bz = 0; emu_bz < grid.z; emu_bz++)
+        for (emu_by = 0; emu_by < grid.y; emu_by++)
+            for (emu_bx = 0; emu_bx < grid.x; emu_bx++) {
+                for (unsigned t = 0; t < nt; t++) {
+                    getcontext(&emu_fib.ctx[t]);
+                    emu_fib.ctx[t].uc_stack.ss_sp = emu_fib.stacks[t];
+                    emu_fib.ctx[t].uc_stack.ss_size = EmuFibers::STACK;
+                    emu_fib.ctx[t].uc_link = nullptr;
+                    makecontext(&emu_fib.ctx[t], (void (*)())emu_fiber_entry, 0);
+                    emu_fib.state[t] = 0;
+                }
+                for (;;) {
+                    bool any = false;
+                    for (unsigned t = 0; t < nt; t++) {
+                        if (emu_fib.state[t] == 2) continue;
+                        any = true;
+                        emu_fib.cur = t;
+                        emu_fib.state[t] = 0;
+                        emu_set_ids(t);
+                        swapcontext(&emu_fib.sched, &emu_fib.ctx[t]);
+                    }
+                    if (!any) break;
+                }
+            }
+}
+#endif
+
 #define hipLaunchKernelGGL(F, G, B, SH, ST, ...) emu_launch(F, dim3(G), dim3(B), __VA_ARGS__)
