@@ -32,7 +32,12 @@ def parse_args():
     ap.add_argument("--n", type=int, default=256, help="grid points per direction")
     ap.add_argument("--order", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-storage run")
+    ap.add_argument("--err-steps", type=int, default=20, help="steps of the FP32-vs-FP64 error comparison")
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0: host cores, at most 16)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (reference point for the "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE counters) and exit")
     return ap.parse_args()
 
 
@@ -76,8 +81,26 @@ def cpu_baseline(n, order, threads):
                       f"({dt:.2f} s)"}
 
 
+def copy_calibration(mib, reps=5):
+    """A plain copy kernel of known size: its FETCH_SIZE / WRITE_SIZE per
+    dispatch, against mib MiB read and written, scales the counters read for
+    the solver's kernels."""
+    import torch
+    n = mib * (1 << 20) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 8, "dispatches": reps}), flush=True)
+
+
 def main():
     a = parse_args()
+    if a.copy_calibration:
+        copy_calibration(a.copy_calibration)
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
         print("bench.py: this build runs on one GPU; --gpus N > 1 is not supported (each GPU box has one GPU)",
@@ -86,37 +109,66 @@ def main():
     import cudanavierstokes_amd as cns
 
     n, order = a.n, a.order
-    cfg = cns.taylor_green(n=n, order=order, scheme="rk4", cfl=0.5, check_cfl=10, check_bulk=10, Re=1600.0,
-                           Pr=0.71, Ma=0.1)
-    sol = cns.Solver(cfg)
-    sol.set_state(*sol.taylor_green_fields())
-    if a.warmup:
-        sol.step(a.warmup, outputs=False)
-    sol.sync()
-    # timed region: HIP events on the solver stream around exactly `steps` steps
-    t0 = time.perf_counter()
-    ms = sol.time_steps(a.steps)
-    wall = time.perf_counter() - t0
-    # per-launch durations of the fused RHS + stage-update kernel
-    sol.profile_enable(True)
-    sol.time_steps(a.steps)
-    kms, launches = sol.profile_read()
-    sol.profile_enable(False)
     pts = n ** 3
-    value = pts * a.steps / (ms / 1e3) / 1e9
-    per_launch_ms = kms / max(launches, 1)
-    bytes_per_launch = RK4_WORDS * 8 * pts / 4.0
-    achieved = bytes_per_launch / (per_launch_ms / 1e3) / 1e9
+    base = cns.taylor_green(n=n, order=order, scheme="rk4", cfl=0.5, check_cfl=10, check_bulk=10, Re=1600.0,
+                            Pr=0.71, Ma=0.1)
+
+    def measure(cfg, wsize):
+        """Throughput of `steps` RK4 steps and the per-launch time of the fused
+        kernel (HIP events on the solver stream)."""
+        sol = cns.Solver(cfg)
+        sol.set_state(*sol.taylor_green_fields())
+        if a.warmup:
+            sol.step(a.warmup, outputs=False)
+        sol.sync()
+        # timed region: HIP events on the solver stream around exactly `steps` steps
+        t0 = time.perf_counter()
+        ms = sol.time_steps(a.steps)
+        wall = time.perf_counter() - t0
+        sol.profile_enable(True)
+        sol.time_steps(a.steps)
+        kms, launches = sol.profile_read()
+        sol.profile_enable(False)
+        sol.close()
+        per_launch_ms = kms / max(launches, 1)
+        bytes_per_launch = RK4_WORDS * wsize * pts / 4.0
+        return {"ms": ms, "wall": wall, "value": pts * a.steps / (ms / 1e3) / 1e9,
+                "per_launch_ms": per_launch_ms, "launches": launches,
+                "achieved": bytes_per_launch / (per_launch_ms / 1e3) / 1e9,
+                "step_gbs": RK4_WORDS * wsize * pts * a.steps / (ms / 1e3) / 1e9}
+
+    def fp32_error(nsteps):
+        """Relative L2 / Linf per conserved variable of the FP32-storage state
+        against the FP64 state after the same nsteps from the same fields."""
+        states = []
+        for cfg in (base, base.replace(precision=1, compensated=1)):
+            sol = cns.Solver(cfg)
+            sol.set_state(*sol.taylor_green_fields())
+            sol.step(nsteps, outputs=False)
+            r, u, v, w, e = sol.get_state()
+            states.append([r, r * u, r * v, r * w, e])
+            sol.close()
+        ref, got = states
+        mom = sum(float((ref[q].astype("float64") ** 2).sum()) for q in (1, 2, 3)) ** 0.5
+        mom_inf = max(float(abs(ref[q]).max()) for q in (1, 2, 3))
+        out = {}
+        for q, nm in enumerate(["r", "ru", "rv", "rw", "re"]):
+            d = got[q] - ref[q]
+            n2 = mom if q in (1, 2, 3) else float((ref[q] ** 2).sum()) ** 0.5
+            ni = mom_inf if q in (1, 2, 3) else float(abs(ref[q]).max())
+            out[nm] = [float((d ** 2).sum()) ** 0.5 / n2, float(abs(d).max()) / ni]
+        return out
+
+    m64 = measure(base, 8)
     traffic = read_traffic()
-    step_gbs = RK4_WORDS * 8 * pts * a.steps / (ms / 1e3) / 1e9
     out = {
         "metric": METRIC,
-        "value": round(value, 4),
+        "value": round(m64["value"], 4),
         "unit": "Gpt-RKsteps/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(ms / a.steps, 4),
+        "ms_per_step": round(m64["ms"] / a.steps, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -125,14 +177,26 @@ def main():
         "config": {"workload": f"Taylor-Green vortex {n}^3, order {order}, classic RK4, CFL 0.5, FP64 storage",
                    "grid": [n, n, n], "order": order, "scheme": "rk4", "cfl": 0.5, "precision": "fp64",
                    "Re": 1600, "Ma": 0.1},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+        "roofline": {"bound": "hbm", "achieved": round(m64["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(m64["achieved"] / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_rhs (fused RHS + RK4 stage update)",
-                     "kernel_ms_per_launch": round(per_launch_ms, 4), "launches": launches},
-        "step_hbm_frac": round(step_gbs / HBM_PEAK_GBS, 4),
-        "wall_s": round(wall, 3),
+                     "kernel_ms_per_launch": round(m64["per_launch_ms"], 4), "launches": m64["launches"]},
+        "step_hbm_frac": round(m64["step_gbs"] / HBM_PEAK_GBS, 4),
+        "wall_s": round(m64["wall"], 3),
     }
-    sol.close()
+    if not a.no_fp32:
+        m32 = measure(base.replace(precision=1, compensated=1), 4)
+        out["fp32"] = {
+            "workload": "same case, FP32 storage (offsets from the mean state), compensated RK sums, "
+                        "FP64 metrics and reductions",
+            "value": round(m32["value"], 4), "unit": "Gpt-RKsteps/s",
+            "ms_per_step": round(m32["ms"] / a.steps, 4),
+            "speedup_vs_fp64": round(m32["value"] / m64["value"], 3),
+            "roofline": {"bound": "hbm", "achieved": round(m32["achieved"], 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(m32["achieved"] / HBM_PEAK_GBS, 4),
+                         "kernel_ms_per_launch": round(m32["per_launch_ms"], 4)},
+            "rel_err_vs_fp64": {"steps": a.err_steps, "l2_linf": fp32_error(a.err_steps)},
+        }
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(n, order, a.cpu_threads or host_cores())
     print(json.dumps(out), flush=True)

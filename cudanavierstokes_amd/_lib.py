@@ -23,7 +23,7 @@ class CnsConfig(C.Structure):
         ("stencil_size", C.c_int), ("stencil_visc", C.c_int),
         ("periodic_x", C.c_int), ("nonuniform_x", C.c_int), ("forcing_z", C.c_int),
         ("sponge", C.c_int), ("scheme", C.c_int), ("check_cfl", C.c_int),
-        ("check_bulk", C.c_int), ("fix_y_work", C.c_int), ("precision", C.c_int),
+        ("check_bulk", C.c_int), ("fix_y_work", C.c_int), ("precision", C.c_int), ("compensated", C.c_int),
         ("device", C.c_int),
         ("len_x", C.c_double), ("len_y", C.c_double), ("len_z", C.c_double),
         ("reynolds", C.c_double), ("prandtl", C.c_double), ("mach", C.c_double), ("gamma", C.c_double),

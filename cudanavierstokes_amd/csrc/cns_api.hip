@@ -106,10 +106,16 @@ struct cns_solver {
     double *d_ref = nullptr;  // 5 sponge reference fields, mx*mz each
     Scalars *d_sc = nullptr;
     double *d_part = nullptr;
-    double *d_dil = nullptr;
+    // state storage: FP64, or FP32 offsets from (rho0, E0) with optional
+    // compensation (low) parts
+    bool fp32 = false, comp = false;
+    size_t esz = 8;
+    double rho0 = 0.0, E0 = 0.0;
+    void *d_dil = nullptr;
     int nsets = 0, nrhs = 0;
-    double *sets[4] = {nullptr};  // conserved-field sets, 5*n each
-    double *rbuf[2] = {nullptr};  // right-hand-side sets, 5*n each
+    void *sets[4] = {nullptr};  // conserved-field sets, 5*n each
+    void *lo[4] = {nullptr};    // FP32 compensation parts of the sets
+    void *rbuf[2] = {nullptr};  // right-hand-side sets, 5*n each
     int cur = 0;
     double *d_par1 = nullptr, *d_par2 = nullptr, *d_time = nullptr;
     int par_cap = 0;
@@ -163,6 +169,12 @@ static int fail(cns_solver *s, int code, const std::string &msg) {
 static int dalloc(cns_solver *s, double **p, size_t count) {
     HIPCHK(s, hipMalloc((void **)p, count * sizeof(double)));
     s->dev_bytes += count * sizeof(double);
+    return CNS_OK;
+}
+
+static int balloc(cns_solver *s, void **p, size_t bytes) {
+    HIPCHK(s, hipMalloc(p, bytes));
+    s->dev_bytes += bytes;
     return CNS_OK;
 }
 
@@ -293,6 +305,7 @@ static void fill_params(cns_solver *s) {
     P.dt = &s->d_sc->dt;
     P.dpdz = &s->d_sc->dpdz;
     P.visc_max_out = &s->d_sc->visc_max;
+    P.comp = s->comp ? 1 : 0;
 }
 
 extern "C" int cns_config_init(cns_config *c) {
@@ -310,6 +323,7 @@ extern "C" int cns_config_init(cns_config *c) {
     c->check_bulk = 10;
     c->fix_y_work = 0;
     c->precision = CNS_FP64;
+    c->compensated = 1;
     c->device = 0;
     c->len_x = c->len_y = c->len_z = 2.0 * M_PI;
     c->reynolds = 1600.0;
@@ -345,6 +359,11 @@ static int validate(const cns_config *c, std::string &msg) {
         msg = "grid larger than 2^31-1 points";
         return CNS_ERR_ARG;
     }
+    if ((long)c->nx * c->ny > (1L << 29)) {
+        // the kernels address a plane with 32-bit byte offsets
+        msg = "x-y plane larger than 2^29 points";
+        return CNS_ERR_ARG;
+    }
     if (c->scheme < 0 || c->scheme > 2) {
         msg = "scheme must be CNS_RK3_LOWSTORAGE, CNS_RK3_KUTTA or CNS_RK4";
         return CNS_ERR_ARG;
@@ -358,9 +377,9 @@ static int validate(const cns_config *c, std::string &msg) {
         msg = "lengths, Re, Pr, Ma, cfl must be positive and gam > 1";
         return CNS_ERR_ARG;
     }
-    if (c->precision != CNS_FP64) {
-        msg = "precision: only CNS_FP64 is available in this build";
-        return CNS_ERR_UNSUPPORTED;
+    if (c->precision != CNS_FP64 && c->precision != CNS_FP32) {
+        msg = "precision must be CNS_FP64 or CNS_FP32";
+        return CNS_ERR_ARG;
     }
     return CNS_OK;
 }
@@ -380,11 +399,16 @@ extern "C" int cns_destroy(cns_solver *s) {
     fr(s->d_ref);
     if (s->d_sc) (void)hipFree(s->d_sc);
     fr(s->d_part);
-    fr(s->d_dil);
-    for (int a = 0; a < 4; a++)
-        fr(s->sets[a]);
+    auto fv = [](void *p) {
+        if (p) (void)hipFree(p);
+    };
+    fv(s->d_dil);
+    for (int a = 0; a < 4; a++) {
+        fv(s->sets[a]);
+        fv(s->lo[a]);
+    }
     for (int a = 0; a < 2; a++)
-        fr(s->rbuf[a]);
+        fv(s->rbuf[a]);
     fr(s->d_par1);
     fr(s->d_par2);
     fr(s->d_time);
@@ -415,6 +439,9 @@ extern "C" int cns_create(const cns_config *cfg, cns_solver **out) {
     }
     s->S = cfg->stencil_size;
     s->SV = cfg->stencil_visc;
+    s->fp32 = cfg->precision == CNS_FP32;
+    s->comp = s->fp32 && cfg->compensated != 0;
+    s->esz = s->fp32 ? sizeof(float) : sizeof(double);
     s->n = (long)cfg->nx * cfg->ny * cfg->nz;
     fd_weights(s->S, s->cF, s->cS);
     fd_weights(s->SV, s->cVF, s->cVS);
@@ -441,13 +468,19 @@ extern "C" int cns_create(const cns_config *cfg, cns_solver **out) {
     s->nrhs = (cfg->scheme == CNS_RK4) ? 1 : 2;
     if ((rc = dalloc(s, &s->d_xp, mx)) || (rc = dalloc(s, &s->d_vsx, s->vsx.size())) ||
         (rc = dalloc(s, &s->d_dxv, mx)) || (rc = dalloc(s, &s->d_spx, mx)) || (rc = dalloc(s, &s->d_spz, mz)) ||
-        (rc = dalloc(s, &s->d_part, (size_t)NPART * RED_BLOCKS)) || (rc = dalloc(s, &s->d_dil, n)))
+        (rc = dalloc(s, &s->d_part, (size_t)NPART * RED_BLOCKS)) || (rc = balloc(s, &s->d_dil, n * s->esz)))
         return bail(rc);
     if ((rc = dalloc(s, &s->d_ref, (size_t)5 * mx * mz))) return bail(rc);
-    for (int a = 0; a < s->nsets; a++)
-        if ((rc = dalloc(s, &s->sets[a], 5 * (size_t)n))) return bail(rc);
+    for (int a = 0; a < s->nsets; a++) {
+        if ((rc = balloc(s, &s->sets[a], 5 * (size_t)n * s->esz))) return bail(rc);
+        // low parts only where a running sum lives: both SMR sets (they
+        // alternate), the Kutta step state (set 0), the RK4 state and
+        // accumulator (sets 0 and 2); intermediate stage inputs need none
+        const bool need = cfg->scheme == CNS_RK3_LOWSTORAGE || a == 0 || (cfg->scheme == CNS_RK4 && a == 2);
+        if (s->comp && need && (rc = balloc(s, &s->lo[a], 5 * (size_t)n * s->esz))) return bail(rc);
+    }
     for (int a = 0; a < s->nrhs; a++)
-        if ((rc = dalloc(s, &s->rbuf[a], 5 * (size_t)n))) return bail(rc);
+        if ((rc = balloc(s, &s->rbuf[a], 5 * (size_t)n * s->esz))) return bail(rc);
     if (hipMalloc((void **)&s->d_sc, sizeof(Scalars)) != hipSuccess) {
         s->err = "hipMalloc(scalars) failed";
         return bail(CNS_ERR_NOMEM);
@@ -544,40 +577,90 @@ extern "C" int cns_fill_channel(const cns_solver *s, uint64_t seed, double ubulk
     return CNS_OK;
 }
 
-// Pointers a stage does not use still point at valid device memory (the
-// stage input), so no kernel argument is ever a null device address.
-static void fill_unused(StagePtrs<double> &A) {
-    double *in = const_cast<double *>(A.qin);
-    if (!A.qout) A.qout = in;
-    if (!A.rout) A.rout = in;
-    if (!A.acc) A.acc = in;
-    if (!A.ra) A.ra = in;
-    if (!A.rb) A.rb = in;
-    if (!A.q0) A.q0 = in;
+template <typename R>
+static R *as(void *p) {
+    return static_cast<R *>(p);
 }
 
-static StagePtrs<double> ptrs_in(cns_solver *s, int set) {
-    StagePtrs<double> A;
+// Pointers a stage does not use still point at valid device memory (the
+// stage input), so no kernel argument is ever a null device address.
+template <typename R>
+static void fill_unused(StagePtrs<R> &A) {
+    R *in = const_cast<R *>(A.qin);
+    if (!A.s1) A.s1 = in;
+    if (!A.ra) A.ra = in;
+    if (!A.rb) A.rb = in;
+    if (!A.s2) A.s2 = in;
+    if (!A.qout) A.qout = in;
+    if (!A.rout) A.rout = in;
+    if (!A.accout) A.accout = in;
+    if (!A.s1lo) A.s1lo = in;
+    if (!A.s2lo) A.s2lo = in;
+    if (!A.qoutlo) A.qoutlo = in;
+    if (!A.accoutlo) A.accoutlo = in;
+}
+
+template <typename R>
+static StagePtrs<R> ptrs_in(cns_solver *s, int set) {
+    StagePtrs<R> A;
     std::memset(&A, 0, sizeof(A));
-    A.qin = s->sets[set];
+    A.qin = as<R>(s->sets[set]);
     A.fs = s->n;
-    A.dil = s->d_dil;
+    A.dil = as<R>(s->d_dil);
     fill_unused(A);
     return A;
+}
+
+// FP32 storage: the reference state of the offsets is the mean density and
+// total energy of the state given to cns_set_state.
+static void set_offsets(cns_solver *s, double rho0, double E0) {
+    const double g = s->cfg.gamma;
+    s->rho0 = rho0;
+    s->E0 = E0;
+    Params &P = s->base;
+    P.rho0 = rho0;
+    P.E0 = E0;
+    const double p0 = (g - 1.0) * E0;
+    P.T0 = p0 / (rho0 * s->Rgas);
+    P.h0 = (E0 + p0) / rho0;
 }
 
 extern "C" int cns_set_state(cns_solver *s, const double *r, const double *u, const double *v, const double *w,
                              const double *e) {
     if (!s || !r || !u || !v || !w || !e) return fail(s, CNS_ERR_ARG, "cns_set_state: null argument");
     HIPCHK(s, hipSetDevice(s->dev));
-    const size_t b = s->n * sizeof(double);
-    const double *src[5] = {r, u, v, w, e};
+    const long n = s->n;
     s->cur = 0;
-    for (int q = 0; q < 5; q++)
-        HIPCHK(s, hipMemcpyAsync(s->sets[0] + q * s->n, src[q], b, hipMemcpyHostToDevice, s->stream));
-    hipLaunchKernelGGL(k_to_conserved<double>, dim3(1024), dim3(256), 0, s->stream, s->n, s->sets[0],
-                       s->sets[0] + s->n, s->sets[0] + 2 * s->n, s->sets[0] + 3 * s->n);
-    LAUNCHED(s, "k_to_conserved<double>");
+    if (!s->fp32) {
+        double *const set = as<double>(s->sets[0]);
+        const size_t b = n * sizeof(double);
+        const double *src[5] = {r, u, v, w, e};
+        for (int q = 0; q < 5; q++)
+            HIPCHK(s, hipMemcpyAsync(set + q * n, src[q], b, hipMemcpyHostToDevice, s->stream));
+        hipLaunchKernelGGL(k_to_conserved<double>, dim3(1024), dim3(256), 0, s->stream, n, set, set + n, set + 2 * n,
+                           set + 3 * n);
+        LAUNCHED(s, "k_to_conserved<double>");
+    } else {
+        // conserved variables in FP64 as the FP64 path forms them (rho u = u rho),
+        // minus the offsets, split into hi = fl(x) and lo = fl(x - hi)
+        double sr = 0.0, se = 0.0;
+        for (long g = 0; g < n; g++) {
+            sr += r[g];
+            se += e[g];
+        }
+        set_offsets(s, sr / n, se / n);
+        std::vector<float> hi(5 * (size_t)n), lo(5 * (size_t)n);
+        for (long g = 0; g < n; g++) {
+            const double x[5] = {r[g] - s->rho0, u[g] * r[g], v[g] * r[g], w[g] * r[g], e[g] - s->E0};
+            for (int q = 0; q < 5; q++) {
+                const float h = (float)x[q];
+                hi[q * n + g] = h;
+                lo[q * n + g] = (float)(x[q] - (double)h);
+            }
+        }
+        HIPCHK(s, hipMemcpy(s->sets[0], hi.data(), hi.size() * sizeof(float), hipMemcpyHostToDevice));
+        if (s->comp) HIPCHK(s, hipMemcpy(s->lo[0], lo.data(), lo.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
     // time restarts at zero (initDevice, src/cuda_utils.cu:386)
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(1), 0, s->stream, &s->d_sc->time, 1L, 0.0);
     LAUNCHED(s, "k_fill");
@@ -591,32 +674,49 @@ extern "C" int cns_get_state(cns_solver *s, double *r, double *u, double *v, dou
     if (!s || !r || !u || !v || !w || !e) return fail(s, CNS_ERR_ARG, "cns_get_state: null argument");
     if (!s->have_state) return fail(s, CNS_ERR_STATE, "cns_get_state: no state set");
     HIPCHK(s, hipSetDevice(s->dev));
-    const size_t b = s->n * sizeof(double);
     const long n = s->n;
-    double *const set = s->sets[s->cur];
-    double *const q[5] = {set, set + n, set + 2 * n, set + 3 * n, set + 4 * n};
-    // velocities into the dilatation buffer and the first rhs buffer
-    double *tu = s->rbuf[0], *tv = s->rbuf[0] + n, *tw = s->rbuf[0] + 2 * n;
-    hipLaunchKernelGGL(k_to_velocity<double>, dim3(1024), dim3(256), 0, s->stream, s->n, q[0], q[1], q[2], q[3], tu,
-                       tv, tw);
-    LAUNCHED(s, "k_to_velocity<double>");
-    HIPCHK(s, hipMemcpyAsync(r, q[0], b, hipMemcpyDeviceToHost, s->stream));
-    HIPCHK(s, hipMemcpyAsync(u, tu, b, hipMemcpyDeviceToHost, s->stream));
-    HIPCHK(s, hipMemcpyAsync(v, tv, b, hipMemcpyDeviceToHost, s->stream));
-    HIPCHK(s, hipMemcpyAsync(w, tw, b, hipMemcpyDeviceToHost, s->stream));
-    HIPCHK(s, hipMemcpyAsync(e, q[4], b, hipMemcpyDeviceToHost, s->stream));
+    if (!s->fp32) {
+        const size_t b = n * sizeof(double);
+        double *const set = as<double>(s->sets[s->cur]);
+        double *const q[5] = {set, set + n, set + 2 * n, set + 3 * n, set + 4 * n};
+        // velocities into the first rhs buffer
+        double *tu = as<double>(s->rbuf[0]), *tv = tu + n, *tw = tu + 2 * n;
+        hipLaunchKernelGGL(k_to_velocity<double>, dim3(1024), dim3(256), 0, s->stream, n, q[0], q[1], q[2], q[3], tu,
+                           tv, tw);
+        LAUNCHED(s, "k_to_velocity<double>");
+        HIPCHK(s, hipMemcpyAsync(r, q[0], b, hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(s, hipMemcpyAsync(u, tu, b, hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(s, hipMemcpyAsync(v, tv, b, hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(s, hipMemcpyAsync(w, tw, b, hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(s, hipMemcpyAsync(e, q[4], b, hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(s, hipStreamSynchronize(s->stream));
+        return CNS_OK;
+    }
+    std::vector<float> hi(5 * (size_t)n), lo(5 * (size_t)n, 0.0f);
     HIPCHK(s, hipStreamSynchronize(s->stream));
+    HIPCHK(s, hipMemcpy(hi.data(), s->sets[s->cur], hi.size() * sizeof(float), hipMemcpyDeviceToHost));
+    if (s->comp) HIPCHK(s, hipMemcpy(lo.data(), s->lo[s->cur], lo.size() * sizeof(float), hipMemcpyDeviceToHost));
+    for (long g = 0; g < n; g++) {
+        double x[5];
+        for (int q = 0; q < 5; q++) x[q] = (double)hi[q * n + g] + (double)lo[q * n + g];
+        r[g] = x[0] + s->rho0;
+        u[g] = x[1] / r[g];
+        v[g] = x[2] / r[g];
+        w[g] = x[3] / r[g];
+        e[g] = x[4] + s->E0;
+    }
     return CNS_OK;
 }
 
 // dilatation + fused RHS/update of one stage
-static int run_stage(cns_solver *s, int in_set, Params P, StagePtrs<double> A) {
+template <typename R>
+static int run_stage(cns_solver *s, int in_set, Params P, StagePtrs<R> A) {
     const bool wallx = !s->cfg.periodic_x, nonuni = s->cfg.nonuniform_x != 0;
-    A.qin = s->sets[in_set];
+    A.qin = as<R>(s->sets[in_set]);
     A.fs = s->n;
-    A.dil = s->d_dil;
+    A.dil = as<R>(s->d_dil);
     fill_unused(A);
-    HIPCHK(s, launch_dil<double>(s->SV, wallx, nonuni, s->nblocks, s->stream, P, A, s->d_dil));
+    HIPCHK(s, launch_dil<R>(s->SV, wallx, nonuni, s->nblocks, s->stream, P, A, as<R>(s->d_dil)));
     LAUNCHED(s, "k_dil");
     if (P.want_visc_max) {
         hipLaunchKernelGGL(k_reset_visc, dim3(1), dim3(1), 0, s->stream, s->d_sc);
@@ -631,7 +731,7 @@ static int run_stage(cns_solver *s, int in_set, Params P, StagePtrs<double> A) {
         }
         HIPCHK(s, hipEventRecord(s->ev[2 * s->prof_n], s->stream));
     }
-    HIPCHK(s, launch_rhs<double>(s->S, s->SV, wallx, nonuni, s->nblocks, s->stream, P, A));
+    HIPCHK(s, launch_rhs<R>(s->S, s->SV, wallx, nonuni, s->nblocks, s->stream, P, A));
     LAUNCHED(s, "k_rhs");
     if (s->prof) {
         HIPCHK(s, hipEventRecord(s->ev[2 * s->prof_n + 1], s->stream));
@@ -640,42 +740,50 @@ static int run_stage(cns_solver *s, int in_set, Params P, StagePtrs<double> A) {
     return CNS_OK;
 }
 
-extern "C" int cns_rhs(cns_solver *s, double *rr, double *ru, double *rv, double *rw, double *re) {
-    if (!s || !rr || !ru || !rv || !rw || !re) return fail(s, CNS_ERR_ARG, "cns_rhs: null argument");
-    if (!s->have_state) return fail(s, CNS_ERR_STATE, "cns_rhs: no state set");
-    HIPCHK(s, hipSetDevice(s->dev));
+template <typename R>
+static int rhs_t(cns_solver *s, double *const dst[5]) {
     Params P = s->base;
     P.write_q = 0;
     P.write_r = 1;
     P.want_visc_max = 1;
-    StagePtrs<double> A;
+    StagePtrs<R> A;
     std::memset(&A, 0, sizeof(A));
-    A.rout = s->rbuf[0];
-    int rc = run_stage(s, s->cur, P, A);
+    A.rout = as<R>(s->rbuf[0]);
+    int rc = run_stage<R>(s, s->cur, P, A);
     if (rc) return rc;
     s->visc_from_state = false;
-    double *dst[5] = {rr, ru, rv, rw, re};
-    const size_t b = s->n * sizeof(double);
-    for (int q = 0; q < 5; q++)
-        HIPCHK(s, hipMemcpyAsync(dst[q], s->rbuf[0] + q * s->n, b, hipMemcpyDeviceToHost, s->stream));
-    HIPCHK(s, hipStreamSynchronize(s->stream));
+    const long n = s->n;
+    if (sizeof(R) == sizeof(double)) {
+        for (int q = 0; q < 5; q++)
+            HIPCHK(s, hipMemcpyAsync(dst[q], as<double>(s->rbuf[0]) + q * n, n * sizeof(double),
+                                     hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(s, hipStreamSynchronize(s->stream));
+    } else {
+        std::vector<R> h(5 * (size_t)n);
+        HIPCHK(s, hipStreamSynchronize(s->stream));
+        HIPCHK(s, hipMemcpy(h.data(), s->rbuf[0], h.size() * sizeof(R), hipMemcpyDeviceToHost));
+        for (int q = 0; q < 5; q++)
+            for (long g = 0; g < n; g++) dst[q][g] = (double)h[q * n + g];
+    }
     return CNS_OK;
 }
 
+extern "C" int cns_rhs(cns_solver *s, double *rr, double *ru, double *rv, double *rw, double *re) {
+    if (!s || !rr || !ru || !rv || !rw || !re) return fail(s, CNS_ERR_ARG, "cns_rhs: null argument");
+    if (!s->have_state) return fail(s, CNS_ERR_STATE, "cns_rhs: no state set");
+    HIPCHK(s, hipSetDevice(s->dev));
+    double *const dst[5] = {rr, ru, rv, rw, re};
+    return s->fp32 ? rhs_t<float>(s, dst) : rhs_t<double>(s, dst);
+}
+
 // Launch parameters of one stage update
-// q_out = w_qc q_in + w_q0 q0 + w_acc acc + dt (w_ra ra + w_rb rb + w_r rhs).
-static Params stage_params(const cns_solver *s, double w_qc, double w_q0, double w_acc, double w_ra, double w_rb,
-                           double w_r) {
+// q_out = s1 + dt (w_ra ra + w_rb rb + w_r rhs).
+static Params stage_params(const cns_solver *s, double w_ra, double w_rb, double w_r) {
     Params P = s->base;
     P.write_q = 1;
-    P.w_qc = w_qc;
-    P.w_q0 = w_q0;
-    P.w_acc = w_acc;
     P.w_ra = w_ra;
     P.w_rb = w_rb;
     P.w_r = w_r;
-    P.use_q0 = w_q0 != 0.0;
-    P.use_acc = w_acc != 0.0;
     P.use_ra = w_ra != 0.0;
     P.use_rb = w_rb != 0.0;
     return P;
@@ -685,12 +793,13 @@ static const double SMR_ALPHA[3] = {0., -17. / 60., -5. / 12.};
 static const double SMR_BETA[3] = {8. / 15., 5. / 12., 3. / 4.};
 
 // queue nsteps steps on the stream; outputs stay on the device
-static int queue_steps(cns_solver *s, int nsteps, bool want_par, bool want_time) {
+template <typename R>
+static int queue_steps_t(cns_solver *s, int nsteps, bool want_par, bool want_time) {
     const cns_config &c = s->cfg;
     const long n = s->n;
     (void)n;
     for (int istep = 0; istep < nsteps; istep++) {
-        StagePtrs<double> cur = ptrs_in(s, s->cur);
+        StagePtrs<R> cur = ptrs_in<R>(s, s->cur);
         const bool cfl_step = (istep % c.check_cfl) == 0;
         const bool bulk_step = want_par && (istep % c.check_bulk) == 0;
         if (cfl_step) {
@@ -699,18 +808,18 @@ static int queue_steps(cns_solver *s, int nsteps, bool want_par, bool want_time)
                 hipLaunchKernelGGL(k_reset_visc, dim3(1), dim3(1), 0, s->stream, s->d_sc);
                 LAUNCHED(s, "k_reset_visc");
             }
-            hipLaunchKernelGGL(k_cfl<double>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s->stream, s->base, cur, vfs,
+            hipLaunchKernelGGL(k_cfl<R>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s->stream, s->base, cur, vfs,
                                s->d_sc);
-            LAUNCHED(s, "k_cfl<double>");
+            LAUNCHED(s, "k_cfl<R>");
             hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(1), 0, s->stream, c.cfl, s->d_sc);
             LAUNCHED(s, "k_dt_finalize");
             s->visc_from_state = false;
         }
         const int what = ((cfl_step && c.forcing_z) ? 1 : 0) | (bulk_step ? 2 : 0);
         if (what) {
-            hipLaunchKernelGGL(k_partials<double>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s->stream, s->base, cur,
+            hipLaunchKernelGGL(k_partials<R>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s->stream, s->base, cur,
                                s->d_part);
-            LAUNCHED(s, "k_partials<double>");
+            LAUNCHED(s, "k_partials<R>");
         }
         if (what & 1) {
             hipLaunchKernelGGL(k_final, dim3(1), dim3(RED_THREADS), 0, s->stream, s->d_part, s->d_sc, 1, c.forcing_z,
@@ -732,14 +841,19 @@ static int queue_steps(cns_solver *s, int nsteps, bool want_par, bool want_time)
             const int a = s->cur, b = 1 - s->cur;
             const int in[3] = {a, b, a}, outs[3] = {b, a, b};
             for (int st = 0; st < 3; st++) {
-                Params P = stage_params(s, /*w_qc*/ 1, 0, 0, /*w_ra*/ st ? SMR_ALPHA[st] : 0, 0, SMR_BETA[st]);
+                Params P = stage_params(s, st ? SMR_ALPHA[st] : 0, 0, SMR_BETA[st]);
                 P.write_r = st < 2;
                 P.want_visc_max = (st == 2 && need_visc) ? 1 : 0;
-                StagePtrs<double> A{};
-                A.qout = s->sets[outs[st]];
-                A.rout = st < 2 ? s->rbuf[st] : nullptr;
-                A.ra = st == 0 ? nullptr : s->rbuf[st - 1];
-                if ((rc = run_stage(s, in[st], P, A))) return rc;
+                StagePtrs<R> A{};
+                A.s1 = as<R>(s->sets[in[st]]);
+                A.qout = as<R>(s->sets[outs[st]]);
+                A.rout = st < 2 ? as<R>(s->rbuf[st]) : nullptr;
+                A.ra = st == 0 ? nullptr : as<R>(s->rbuf[st - 1]);
+                // every stage output is the next stage's running sum
+                A.s1lo = as<R>(s->lo[in[st]]);
+                A.qoutlo = as<R>(s->lo[outs[st]]);
+                P.write_qlo = 1;
+                if ((rc = run_stage<R>(s, in[st], P, A))) return rc;
             }
             s->cur = b;
         } else if (c.scheme == CNS_RK3_KUTTA) {
@@ -748,18 +862,22 @@ static int queue_steps(cns_solver *s, int nsteps, bool want_par, bool want_time)
             const int in[3] = {q0, s1, s2}, outs[3] = {s1, s2, q0};
             for (int st = 0; st < 3; st++) {
                 Params P;
-                if (st == 0) P = stage_params(s, 1, 0, 0, 0, 0, 0.5);
-                else if (st == 1) P = stage_params(s, 0, 1, 0, -1.0, 0, 2.0);
-                else P = stage_params(s, 0, 1, 0, 1.0 / 6.0, 4.0 / 6.0, 1.0 / 6.0);
+                if (st == 0) P = stage_params(s, 0, 0, 0.5);
+                else if (st == 1) P = stage_params(s, -1.0, 0, 2.0);
+                else P = stage_params(s, 1.0 / 6.0, 4.0 / 6.0, 1.0 / 6.0);
                 P.write_r = st < 2;
                 P.want_visc_max = (st == 2 && need_visc) ? 1 : 0;
-                StagePtrs<double> A{};
-                A.qout = s->sets[outs[st]];
-                A.q0 = s->sets[q0];
-                A.rout = st < 2 ? s->rbuf[st] : nullptr;
-                A.ra = s->rbuf[0];
-                A.rb = s->rbuf[1];
-                if ((rc = run_stage(s, in[st], P, A))) return rc;
+                StagePtrs<R> A{};
+                A.qout = as<R>(s->sets[outs[st]]);
+                A.s1 = as<R>(s->sets[q0]);  // the stage-0 input is q0 itself
+                A.rout = st < 2 ? as<R>(s->rbuf[st]) : nullptr;
+                A.ra = as<R>(s->rbuf[0]);
+                A.rb = as<R>(s->rbuf[1]);
+                // q1, q2 only feed the RHS; the step result carries a low part
+                A.s1lo = as<R>(s->lo[q0]);
+                A.qoutlo = as<R>(s->lo[outs[st]]);
+                P.write_qlo = st == 2;
+                if ((rc = run_stage<R>(s, in[st], P, A))) return rc;
             }
             s->cur = q0;
         } else {
@@ -769,28 +887,38 @@ static int queue_steps(cns_solver *s, int nsteps, bool want_par, bool want_time)
             const int in[4] = {sa, sb, sd, sb}, outs[4] = {sb, sd, sb, sa};
             for (int st = 0; st < 4; st++) {
                 Params P;
-                if (st == 0) P = stage_params(s, 1, 0, 0, 0, 0, 0.5);
-                else if (st == 1) P = stage_params(s, 0, 1, 0, 0, 0, 0.5);
-                else if (st == 2) P = stage_params(s, 0, 1, 0, 0, 0, 1.0);
-                else P = stage_params(s, 0, 0, 1, 0, 0, 1.0 / 6.0);
+                if (st == 0) P = stage_params(s, 0, 0, 0.5);
+                else if (st == 1) P = stage_params(s, 0, 0, 0.5);
+                else if (st == 2) P = stage_params(s, 0, 0, 1.0);
+                else P = stage_params(s, 0, 0, 1.0 / 6.0);
+                StagePtrs<R> A{};
+                A.qout = as<R>(s->sets[outs[st]]);
+                A.s1 = as<R>(s->sets[st < 3 ? sa : scc]);
+                A.s1lo = as<R>(s->lo[st < 3 ? sa : scc]);
+                // stage inputs only feed the RHS; the step result carries a low part
+                A.qoutlo = as<R>(s->lo[outs[st]]);
+                P.write_qlo = st == 3;
                 if (st < 3) {
+                    // acc = q0 + dt/6 k1, then acc += dt/3 k2, acc += dt/3 k3
                     P.write_acc = 1;
-                    P.use_acc = st > 0;
-                    P.v_qc = st == 0 ? 1.0 : 0.0;
-                    P.v_acc = st == 0 ? 0.0 : 1.0;
                     P.v_r = st == 0 ? 1.0 / 6.0 : 1.0 / 3.0;
+                    A.s2 = as<R>(s->sets[st == 0 ? sa : scc]);
+                    A.accout = as<R>(s->sets[scc]);
+                    A.s2lo = as<R>(s->lo[st == 0 ? sa : scc]);
+                    A.accoutlo = as<R>(s->lo[scc]);
                 }
                 P.want_visc_max = (st == 3 && need_visc) ? 1 : 0;
-                StagePtrs<double> A{};
-                A.qout = s->sets[outs[st]];
-                A.q0 = s->sets[sa];
-                A.acc = s->sets[scc];
-                if ((rc = run_stage(s, in[st], P, A))) return rc;
+                if ((rc = run_stage<R>(s, in[st], P, A))) return rc;
             }
             s->cur = sa;
         }
     }
     return CNS_OK;
+}
+
+static int queue_steps(cns_solver *s, int nsteps, bool want_par, bool want_time) {
+    return s->fp32 ? queue_steps_t<float>(s, nsteps, want_par, want_time)
+                   : queue_steps_t<double>(s, nsteps, want_par, want_time);
 }
 
 static int ensure_par(cns_solver *s, int nsteps) {

@@ -83,9 +83,10 @@ __global__ void k_cfl(Params P, StagePtrs<R> A, int visc_from_state, Scalars *sc
     for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x) {
         const int i = (int)(g % P.mx);
         const long fs = A.fs;
-        const double r = (double)A.qin[g];
+        // full values (the offsets are zero for FP64 storage)
+        const double r = (double)A.qin[g] + P.rho0;
         const double u = (double)A.qin[g + fs] / r, v = (double)A.qin[g + 2 * fs] / r, w = (double)A.qin[g + 3 * fs] / r;
-        const double ien = (double)A.qin[g + 4 * fs] / r - 0.5 * (u * u + v * v + w * w);
+        const double ien = ((double)A.qin[g + 4 * fs] + P.E0) / r - 0.5 * (u * u + v * v + w * w);
         const double sos = sqrt(P.gam * (P.gam - 1) * ien);
         const double dx = P.dxv[i];
         const double c1 = (fabs(u) + sos) / dx, c2 = (fabs(v) + sos) * P.d_dy, c3 = (fabs(w) + sos) * P.d_dz;
@@ -135,12 +136,12 @@ __global__ void k_partials(Params P, StagePtrs<R> A, double *part) {
         const int i = (int)(g % P.mx);
         const double wgt = P.dxv[i];
         const long fs = A.fs;
-        const double r = (double)A.qin[g];
+        const double r = (double)A.qin[g] + P.rho0;
         const double ru = (double)A.qin[g + fs], rv = (double)A.qin[g + 2 * fs], rw = (double)A.qin[g + 3 * fs];
         const double u = ru / r, v = rv / r, w = rw / r;
         s[0] += r * wgt / P.d_dy / P.d_dz;
         s[1] += rw * wgt / P.d_dy / P.d_dz;
-        s[2] += (double)A.qin[g + 4 * fs] * wgt / P.d_dy / P.d_dz;
+        s[2] += ((double)A.qin[g + 4 * fs] + P.E0) * wgt / P.d_dy / P.d_dz;
         s[3] += (u * u + v * v + w * w) * wgt / P.d_dy / P.d_dz;
     }
     __shared__ double sh[NPART][RED_THREADS];

@@ -24,15 +24,21 @@ struct Params {
     int forcing, sponge, fix_y_work, unit_visc;
     int want_visc_max;  // accumulate max(mu/dx^2, ...) for the next CFL check
     int xcd_remap;
-    // stage update, one linear combination per conserved field:
-    //   q_out = w_qc q_in + w_q0 q0 + w_acc acc + dt (w_ra ra + w_rb rb + w_r rhs)
-    //   acc   = v_qc q_in + v_acc acc + dt (v_r rhs)        (when write_acc)
-    //   r_out = rhs                                          (when write_r)
-    int use_q0, use_acc, use_ra, use_rb, write_q, write_r, write_acc;
-    double w_qc, w_q0, w_acc, w_ra, w_rb, w_r, v_qc, v_acc, v_r;
+    // stage update, per conserved field:
+    //   q_out   = s1 + dt (w_ra ra + w_rb rb + w_r rhs)   (when write_q)
+    //   acc_out = s2 + dt (v_r rhs)                       (when write_acc)
+    //   r_out   = rhs                                     (when write_r)
+    int write_q, use_ra, use_rb, write_acc, write_r;
+    int comp;       // FP32 storage: the state carries a compensation (low) part
+    int write_qlo;  // FP32 storage: q_out gets its low part (the step result)
+    double w_ra, w_rb, w_r, v_r;
     double d_dx, d_dy, d_dz, d_d2x, d_d2y, d_d2z;
     double Rgas, gam, cvInv, Re, Pr, Ec, viscexp, invPrEc, invRe, cpR;
     double twall_bot, twall_top;
+    // FP32 storage keeps rho and rho E as offsets from a reference state
+    // (rho0, E0) with T0 = p0/(rho0 R), h0 = (E0 + p0)/rho0, p0 = (gam-1) E0;
+    // all four are zero for FP64 storage
+    double rho0, E0, T0, h0;
     double cF[4], cS[5], cVF[4], cVS[5];
     const double *xp, *vsx, *dxv, *spx, *spz;
     const double *ref;  // 5 sponge reference fields of mx*mz, contiguous
@@ -46,13 +52,24 @@ template <typename R>
 struct StagePtrs {
     const R *qin;  // stage input (read with stencils)
     const R *dil;  // dilatation of qin (one field)
-    R *qout;       // updated state
-    R *rout;       // right-hand side output
+    const R *s1;   // state term of the update (stage input, step-start state or accumulator)
     const R *ra;   // previous rhs (low-storage RK3 r_prev; Kutta k1)
     const R *rb;   // Kutta k2
-    const R *q0;   // step-start state (Kutta, RK4)
-    R *acc;        // RK4 accumulator
-    long fs;       // field stride (points per field)
+    const R *s2;   // state term of the accumulator update (RK4)
+    R *qout;       // updated state
+    R *rout;       // right-hand side output
+    R *accout;     // accumulator output (RK4)
+    // FP32 storage: low parts of the compensated state (x = hi + lo), one per
+    // state array above (unused with FP64 storage)
+    const R *s1lo, *s2lo;
+    R *qoutlo, *accoutlo;
+    long fs;  // field stride (points per field)
+};
+
+// FP32 storage computes with offsets from the reference state
+template <typename R>
+struct Off {
+    static constexpr bool on = sizeof(R) == 4;
 };
 
 // Central-difference weights of order 2s as compile-time constants (exact
@@ -141,7 +158,7 @@ __device__ __forceinline__ int remap_block(int b, int nb, int on) {
 }
 
 // Equation of state and viscosity law (src/cuda_main.cu:228-239).
-template <typename R>
+template <typename R, bool UNITV>
 __device__ __forceinline__ void eos(const Params &P, R r, R ru, R rv, R rw, R re, R &u, R &v, R &w, R &t,
                                     R &h, R &mu) {
     R inv = R(1) / r;
@@ -153,30 +170,66 @@ __device__ __forceinline__ void eos(const Params &P, R r, R ru, R rv, R rw, R re
     t = R(P.cvInv) * en;
     // total enthalpy (rhoE + p)/rho = cp T + |u|^2/2 with cp = gam R/(gam-1)
     h = t * R(P.cpR) + k2;
-    R suth = P.unit_visc ? t : R(pow((double)t, P.viscexp));
-    mu = suth * R(P.invRe);
+    // mu = T^viscexp / Re; for viscexp = 1 the kernels rebuild it as T/Re
+    mu = UNITV ? R(0) : R(pow((double)t, P.viscexp)) * R(P.invRe);
 }
 
 // Isothermal no-slip wall ghost from its mirror point (src/boundary.h:79-148):
 // velocity antisymmetric, pressure and dilatation symmetric, T = 2Tw - T,
 // mu and lambda from the ghost T, rho = p/(R T), h = cp T + |u|^2/2.
-template <typename R>
+template <typename R, bool UNITV>
 __device__ __forceinline__ void wall_ghost(const Params &P, R tw, R &r, R &u, R &v, R &w, R &t, R &h, R &mu) {
     R p = r * R(P.Rgas) * t;
     u = -u;
     v = -v;
     w = -w;
     t = R(2.0) * tw - t;
-    R suth = P.unit_visc ? t : R(pow((double)t, P.viscexp));
-    mu = suth * R(P.invRe);
-    h = t * R(P.Rgas) * R(P.gam) / (R(P.gam) - R(1.0)) + R(0.5) * (u * u + v * v + w * w);
+    mu = UNITV ? R(0) : R(pow((double)t, P.viscexp)) * R(P.invRe);
+    h = t * R(P.cpR) + R(0.5) * (u * u + v * v + w * w);
     r = p / (R(P.Rgas) * t);
 }
 
-// Quantity indices of the staged arrays.
-enum { QR = 0, QU, QV, QW, QH, QT, QM, QD, NQ };
-// Quantities of the z register window: h is rebuilt from T and u (h = cp T +
-// |u|^2/2) and, for mu = T/Re, mu from T, so the window holds 6 or 7 of them.
-enum { ZR = 0, ZU, ZV, ZW, ZT, ZD, ZM };
+// Equation of state on offset storage (FP32): from rho' = rho - rho0 and
+// E' = rhoE - E0,
+//   p' = (gam-1)(E' - rho |u|^2/2),  T' = (p' - rho' R T0)/(rho R),
+//   h' = (E' + p' - rho' h0)/rho,
+// so that T = T0 + T' and h = h0 + h' carry the fluctuation at full relative
+// precision (p0, T0, h0 cancel exactly in the algebra).
+template <typename R, bool UNITV>
+__device__ __forceinline__ void eos_off(const Params &P, R rp, R ru, R rv, R rw, R ep, R &u, R &v, R &w, R &tp,
+                                        R &hp, R &mu) {
+    const R r = R(P.rho0) + rp;
+    const R inv = R(1) / r;
+    u = ru * inv;
+    v = rv * inv;
+    w = rw * inv;
+    const R k2 = R(0.5) * (u * u + v * v + w * w);
+    const R pp = R(P.gam - 1.0) * (ep - r * k2);
+    tp = (pp - rp * R(P.Rgas * P.T0)) * inv * R(1.0 / P.Rgas);
+    hp = (ep + pp - rp * R(P.h0)) * inv;
+    mu = UNITV ? R(0) : R(pow(P.T0 + (double)tp, P.viscexp) * P.invRe);
+}
+
+// Wall ghost on offset storage, evaluated in FP64 on the full values.
+template <typename R, bool UNITV>
+__device__ __forceinline__ void wall_ghost_off(const Params &P, double tw, R &rp, R &u, R &v, R &w, R &tp, R &hp,
+                                               R &mu) {
+    double r = P.rho0 + (double)rp, t = P.T0 + (double)tp;
+    const double p = r * P.Rgas * t;
+    const double ud = -(double)u, vd = -(double)v, wd = -(double)w;
+    t = 2.0 * tw - t;
+    r = p / (P.Rgas * t);
+    u = R(ud);
+    v = R(vd);
+    w = R(wd);
+    mu = UNITV ? R(0) : R(pow(t, P.viscexp) * P.invRe);
+    hp = R(t * P.cpR + 0.5 * (ud * ud + vd * vd + wd * wd) - P.h0);
+    rp = R(r - P.rho0);
+    tp = R(t - P.T0);
+}
+
+// Quantity indices of the staged primitives (LDS plane and z register
+// window).  For mu = T/Re the last one is not staged: it is rebuilt from T.
+enum { QR = 0, QU, QV, QW, QH, QT, QD, QM, NQ };
 
 }  // namespace cns

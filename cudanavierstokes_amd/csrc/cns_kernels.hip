@@ -86,5 +86,8 @@ hipError_t LAUNCH_DIL(bool wallx, bool nonuni, int nb, hipStream_t st, const Par
 template hipError_t LAUNCH_RHS<double>(int, bool, bool, int, hipStream_t, const Params &, const StagePtrs<double> &);
 template hipError_t LAUNCH_DIL<double>(bool, bool, int, hipStream_t, const Params &, const StagePtrs<double> &,
                                        double *);
+template hipError_t LAUNCH_RHS<float>(int, bool, bool, int, hipStream_t, const Params &, const StagePtrs<float> &);
+template hipError_t LAUNCH_DIL<float>(bool, bool, int, hipStream_t, const Params &, const StagePtrs<float> &,
+                                      float *);
 
 }  // namespace cns

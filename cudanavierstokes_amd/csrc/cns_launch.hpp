@@ -6,8 +6,8 @@
 
 namespace cns {
 
-constexpr int TILE_X = 64;
-constexpr int TILE_Y = 4;
+constexpr int TILE_X = 16;
+constexpr int TILE_Y = 16;
 
 #define CNS_DECLARE_LAUNCH(N)                                                                                  \
     template <typename R>                                                                                      \

@@ -22,14 +22,14 @@
 
 namespace cns {
 
-// Load one point of the stage input, convert to primitives and apply the
-// x ghost rule.  (gi, gj) may lie outside the domain: y wraps periodically;
-// x wraps (periodic) or mirrors to a wall ghost (src/boundary_condition_x.h).
-template <typename R, bool WALLX>
-__device__ __forceinline__ void load_prim(const Params &P, const StagePtrs<R> &A, int gi, int gj, int k,
-                                          R *q) {
+// Source of one staged point: the in-plane byte offset of the point whose
+// value is used (y wraps; x wraps, or mirrors at a wall) and the wall side
+// (-1 bottom, +1 top, 0 none) whose ghost rule then applies
+// (src/boundary_condition_x.h, src/boundary.h:79-148).
+template <bool WALLX>
+__device__ __forceinline__ unsigned src_offset(const Params &P, int gi, int gj, unsigned esz, int &side) {
     gj = wrapi(gj, P.my);
-    int side = 0;
+    side = 0;
     if (WALLX) {
         if (gi < 0) {
             side = -1;
@@ -42,21 +42,55 @@ __device__ __forceinline__ void load_prim(const Params &P, const StagePtrs<R> &A
     } else {
         gi = wrapi(gi, P.mx);
     }
-    const long g = gi + (long)P.mx * (gj + (long)P.my * k);
-    const long fs = A.fs;
-    R r = A.qin[g];
-    R u, v, w, t, h, mu;
-    eos<R>(P, r, A.qin[g + fs], A.qin[g + 2 * fs], A.qin[g + 3 * fs], A.qin[g + 4 * fs], u, v, w, t, h, mu);
-    R d = A.dil[g];
-    if (WALLX && side != 0) wall_ghost<R>(P, side < 0 ? R(P.twall_bot) : R(P.twall_top), r, u, v, w, t, h, mu);
+    return (unsigned)(gi + P.mx * gj) * esz;
+}
+
+// Loads address a uniform plane base plus a 32-bit per-lane byte offset, the
+// form of a global load with a scalar base.
+template <typename R>
+__device__ __forceinline__ R ld_at(const R *base, unsigned boff) {
+    return *reinterpret_cast<const R *>(reinterpret_cast<const char *>(base) + boff);
+}
+template <typename R>
+__device__ __forceinline__ void st_at(R *base, unsigned boff, R v) {
+    *reinterpret_cast<R *>(reinterpret_cast<char *>(base) + boff) = v;
+}
+
+// Conserved values and dilatation of one point, as loaded.
+template <typename R>
+struct Raw {
+    R c[5], d;
+};
+
+template <typename R>
+__device__ __forceinline__ void load_raw(const StagePtrs<R> &A, long poff, unsigned boff, Raw<R> &x) {
+    const R *b = A.qin + poff;
+#pragma unroll
+    for (int q = 0; q < 5; q++) x.c[q] = ld_at(b + q * A.fs, boff);
+    x.d = ld_at(A.dil + poff, boff);
+}
+
+// Primitives of one loaded point with the x ghost rule of its side applied.
+// On offset storage (FP32) rho, T and h are staged as rho', T', h'.
+template <typename R, bool WALLX, bool UNITV>
+__device__ __forceinline__ void to_prim(const Params &P, const Raw<R> &x, int side, R *q) {
+    R r = x.c[0], u, v, w, t, h, mu;
+    if constexpr (Off<R>::on) {
+        eos_off<R, UNITV>(P, r, x.c[1], x.c[2], x.c[3], x.c[4], u, v, w, t, h, mu);
+        if (WALLX && side != 0) wall_ghost_off<R, UNITV>(P, side < 0 ? P.twall_bot : P.twall_top, r, u, v, w, t, h, mu);
+    } else {
+        eos<R, UNITV>(P, r, x.c[1], x.c[2], x.c[3], x.c[4], u, v, w, t, h, mu);
+        if (WALLX && side != 0)
+            wall_ghost<R, UNITV>(P, side < 0 ? R(P.twall_bot) : R(P.twall_top), r, u, v, w, t, h, mu);
+    }
     q[QR] = r;
     q[QU] = u;
     q[QV] = v;
     q[QW] = w;
     q[QH] = h;
     q[QT] = t;
+    q[QD] = x.d;
     q[QM] = mu;
-    q[QD] = d;
 }
 
 // Viscous-order first and second derivatives of u, v, w along DIR
@@ -77,14 +111,26 @@ __device__ __forceinline__ void vel_stencils(const Params &P, Acc f, const doubl
             a1 += R(FD<SV>::a(k)) * (fp - fm);
             a2 += R(FD<SV>::b(k)) * (fp + fm);
         }
-        a1 = a1 * d;
-        if (DIR == 0 && NONUNI) a1 = a1 * R(xpi);
-        d1[c] = a1;
+        if (Off<R>::on && DIR == 0 && NONUNI) {
+            // stretched-grid metric terms in FP64
+            d1[c] = R((double)a1 * (P.d_dx * xpi));
+        } else {
+            a1 = a1 * d;
+            if (DIR == 0 && NONUNI) a1 = a1 * R(xpi);
+            d1[c] = a1;
+        }
         if (DIR == 0 && NONUNI) {
-            R b = R(0);
+            if constexpr (Off<R>::on) {
+                double b = 0.0;
 #pragma unroll
-            for (int it = 0; it < 2 * SV + 1; it++) b += R(vs[it]) * f(q, it - SV);
-            d2[c] = b;
+                for (int it = 0; it < 2 * SV + 1; it++) b += vs[it] * (double)f(q, it - SV);
+                d2[c] = R(b);
+            } else {
+                R b = R(0);
+#pragma unroll
+                for (int it = 0; it < 2 * SV + 1; it++) b += R(vs[it]) * f(q, it - SV);
+                d2[c] = b;
+            }
         } else {
             d2[c] = (a2 + R(FD<SV>::b(0)) * f(q, 0)) * dd;
         }
@@ -93,7 +139,7 @@ __device__ __forceinline__ void vel_stencils(const Params &P, Acc f, const doubl
 
 // Everything else of one direction's part of the RHS; returns the five
 // partial right-hand sides in the reference's term order.
-template <typename R, int S, int SV, int DIR, bool NONUNI, class Acc>
+template <typename R, int S, int SV, int DIR, bool NONUNI, bool UNITV, class Acc>
 __device__ __forceinline__ void dir_terms(const Params &P, Acc f, const double *vs, double xpi, const R g[9],
                                          const R d2v[3], R out[5]) {
     const R d = R(DIR == 0 ? P.d_dx : (DIR == 1 ? P.d_dy : P.d_dz));
@@ -103,32 +149,51 @@ __device__ __forceinline__ void dir_terms(const Params &P, Acc f, const double *
     const R lamc = muc * R(P.invPrEc);
     const R Rg = R(P.Rgas);
     // viscous-order first derivatives of mu, T, dil; second derivative of T
+    // (for mu = T/Re the mu derivative is the T derivative over Re)
     R d1m = R(0), d1t = R(0), d1d = R(0), d2t = R(0);
 #pragma unroll
     for (int k = SV; k >= 1; k--) {
         const R a = R(FD<SV>::a(k));
         const R tp = f(QT, k), tm = f(QT, -k);
-        d1m += a * (f(QM, k) - f(QM, -k));
+        if (!UNITV) d1m += a * (f(QM, k) - f(QM, -k));
         d1t += a * (tp - tm);
         d1d += a * (f(QD, k) - f(QD, -k));
         d2t += R(FD<SV>::b(k)) * (tp + tm);
     }
     R xs = d;
-    if (DIR == 0 && NONUNI) xs = d * R(xpi);
-    d1m = d1m * xs;
+    if (DIR == 0 && NONUNI) xs = Off<R>::on ? R(P.d_dx * xpi) : d * R(xpi);
     d1t = d1t * xs;
+    d1m = UNITV ? d1t * R(P.invRe) : d1m * xs;
     d1d = d1d * xs;
     if (DIR == 0 && NONUNI) {
-        d2t = R(0);
+        if constexpr (Off<R>::on) {
+            double b = 0.0;
 #pragma unroll
-        for (int it = 0; it < 2 * SV + 1; it++) d2t += R(vs[it]) * f(QT, it - SV);
+            for (int it = 0; it < 2 * SV + 1; it++) b += vs[it] * (double)f(QT, it - SV);
+            d2t = R(b);
+        } else {
+            d2t = R(0);
+#pragma unroll
+            for (int it = 0; it < 2 * SV + 1; it++) d2t += R(vs[it]) * f(QT, it - SV);
+        }
     } else {
         d2t = (d2t + R(FD<SV>::b(0)) * f(QT, 0)) * dd;
     }
     // advective-order pressure gradient, p = rho R T
     R d1p = R(0);
+    if constexpr (Off<R>::on) {
+        // p' / R = rho' T + rho0 T' on offset storage
+        const R T0 = R(P.T0), r0 = R(P.rho0);
 #pragma unroll
-    for (int k = S; k >= 1; k--) d1p += R(FD<S>::a(k)) * (f(QR, k) * f(QT, k) - f(QR, -k) * f(QT, -k));
+        for (int k = S; k >= 1; k--) {
+            const R pp = f(QR, k) * (T0 + f(QT, k)) + r0 * f(QT, k);
+            const R pm = f(QR, -k) * (T0 + f(QT, -k)) + r0 * f(QT, -k);
+            d1p += R(FD<S>::a(k)) * (pp - pm);
+        }
+    } else {
+#pragma unroll
+        for (int k = S; k >= 1; k--) d1p += R(FD<S>::a(k)) * (f(QR, k) * f(QT, k) - f(QR, -k) * f(QT, -k));
+    }
     d1p = d1p * (Rg * xs);
     const R d1l = d1m * R(P.invPrEc);
 
@@ -137,11 +202,16 @@ __device__ __forceinline__ void dir_terms(const Params &P, Acc f, const double *
     const int QA = (DIR == 0) ? QU : (DIR == 1 ? QV : QW);
     R fr = R(0), fu = R(0), fv = R(0), fw = R(0), fe = R(0);
     const R rc = f(QR, 0), ac = f(QA, 0), hc = f(QH, 0);
+    // on offset storage rho_a + rho_b = rho'_a + rho'_b + 2 rho0, and h is h'
+    // (the h0 part of the energy flux is h0 times the mass flux, added below)
+    const R r2 = R(2.0 * P.rho0);
 #pragma unroll
     for (int l = S; l >= 1; l--) {
         const R c = R(FD<S>::a(l));
-        const R qp = c * ((rc + f(QR, l)) * (ac + f(QA, l)));
-        const R qm = c * ((f(QR, -l) + rc) * (f(QA, -l) + ac));
+        const R rsp = Off<R>::on ? (rc + f(QR, l)) + r2 : rc + f(QR, l);
+        const R rsm = Off<R>::on ? (f(QR, -l) + rc) + r2 : f(QR, -l) + rc;
+        const R qp = c * (rsp * (ac + f(QA, l)));
+        const R qm = c * (rsm * (f(QA, -l) + ac));
         fr += qp - qm;
         fu += qp * (uc + f(QU, l)) - qm * (f(QU, -l) + uc);
         fv += qp * (vc + f(QV, l)) - qm * (f(QV, -l) + vc);
@@ -149,7 +219,8 @@ __device__ __forceinline__ void dir_terms(const Params &P, Acc f, const double *
         fe += qp * (hc + f(QH, l)) - qm * (f(QH, -l) + hc);
     }
     const R sq = R(-0.5) * xs, sc = R(-0.25) * xs;
-    const R cr = sq * fr, cu = sc * fu, cv = sc * fv, cw = sc * fw, ce = sc * fe;
+    const R cr = sq * fr, cu = sc * fu, cv = sc * fv, cw = sc * fw;
+    const R ce = Off<R>::on ? sc * fe + R(P.h0) * cr : sc * fe;
 
     // gradient index map: 0 dudx 1 dvdx 2 dwdx 3 dudy 4 dvdy 5 dwdy 6 dudz 7 dvdz 8 dwdz
     R su, sv, sw, e;
@@ -219,235 +290,72 @@ __device__ __forceinline__ void block_max_atomic(double v, unsigned long long *o
     }
 }
 
+// Bits of the stage-update flags.
+enum { F_Q = 1, F_RA = 2, F_RB = 4, F_ACC = 8, F_R = 16, F_SPONGE = 32, F_VISC = 64, F_COMP = 128, F_QLO = 256 };
+
+__device__ __forceinline__ long long uniform64(long long v) {
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v >> 32));
+    return (long long)(((unsigned long long)hi << 32) | lo);
+}
+
+// Fused RHS + stage update.  A TX x TY workgroup owns a tile of x-y columns
+// and marches along z over one chunk of planes:
+//   - plane k of the tile plus an x/y halo of width S is staged as primitives
+//     in LDS (double buffered, one barrier per plane) for the x and y
+//     stencils; with TX = TY = 16 and S <= 4 every lane stages at most one
+//     halo point, whose source offset and ghost rule are fixed per lane;
+//   - each lane keeps a (2S+1)-plane register window of its own column for
+//     the z stencils;
+//   - global loads are issued one plane ahead (window feed, halo) or at the
+//     top of the plane (stage-update operands), so their latency overlaps the
+//     stencil arithmetic.
+// With FP32 storage (R = float) the state is stored as offsets from the
+// reference state and the update is evaluated in FP64 on the compensated
+// pair hi + lo, rounded back to hi and lo (F_COMP).
 template <typename R, int S, int SV, int TX, int TY, bool WALLX, bool NONUNI, bool UNITV>
 __global__ __launch_bounds__(TX *TY) void k_rhs(Params P, StagePtrs<R> A) {
     constexpr int H = S, LX = TX + 2 * H, LY = TY + 2 * H, NT = TX * TY, W = 2 * H + 1;
-    constexpr int NZ = UNITV ? 6 : 7;
+    constexpr int NL = UNITV ? NQ - 1 : NQ;  // staged quantities (mu rebuilt from T when mu = T/Re)
     constexpr int NH = 2 * H * TY + 2 * H * TX;
-    __shared__ R sq[NQ][LY * LX];
+    static_assert(NH <= NT, "one halo point per lane");
+    constexpr unsigned ESZ = sizeof(R);
+    __shared__ R sq[2][NL][LY * LX];
     __shared__ double red[NT / 64 + 1];
-    // Stage-update weights, flags and output pointers live in LDS and are read
-    // only by the update, so they do not occupy scalar registers across the
-    // z march (the barriers in the loop keep these loads in place).
-    __shared__ double s_w[9];
-    __shared__ unsigned long long s_ptr[6];
+    // Update weights, flags and the update's arrays (as byte offsets from the
+    // stage input) live in LDS and are read where they are used, so they do
+    // not hold scalar registers across the z march.
+    __shared__ double s_w[4];
+    __shared__ long long s_off[11];
     __shared__ int s_flags;
     if (threadIdx.x == 0) {
-        s_w[0] = P.w_qc;
-        s_w[1] = P.w_q0;
-        s_w[2] = P.w_acc;
-        s_w[3] = P.w_ra;
-        s_w[4] = P.w_rb;
-        s_w[5] = P.w_r;
-        s_w[6] = P.v_qc;
-        s_w[7] = P.v_acc;
-        s_w[8] = P.v_r;
-        s_ptr[0] = (unsigned long long)A.qout;
-        s_ptr[1] = (unsigned long long)A.rout;
-        s_ptr[2] = (unsigned long long)A.ra;
-        s_ptr[3] = (unsigned long long)A.rb;
-        s_ptr[4] = (unsigned long long)A.q0;
-        s_ptr[5] = (unsigned long long)A.acc;
-        s_flags = (P.use_q0 ? 1 : 0) | (P.use_acc ? 2 : 0) | (P.use_ra ? 4 : 0) | (P.use_rb ? 8 : 0) |
-                  (P.write_q ? 16 : 0) | (P.write_r ? 32 : 0) | (P.write_acc ? 64 : 0) | (P.sponge ? 128 : 0) |
-                  (P.want_visc_max ? 256 : 0);
+        s_w[0] = P.w_ra;
+        s_w[1] = P.w_rb;
+        s_w[2] = P.w_r;
+        s_w[3] = P.v_r;
+        const char *q0 = reinterpret_cast<const char *>(A.qin);
+        s_off[0] = reinterpret_cast<const char *>(A.s1) - q0;
+        s_off[1] = reinterpret_cast<const char *>(A.ra) - q0;
+        s_off[2] = reinterpret_cast<const char *>(A.rb) - q0;
+        s_off[3] = reinterpret_cast<const char *>(A.s2) - q0;
+        s_off[4] = reinterpret_cast<const char *>(A.qout) - q0;
+        s_off[5] = reinterpret_cast<const char *>(A.rout) - q0;
+        s_off[6] = reinterpret_cast<const char *>(A.accout) - q0;
+        s_off[7] = reinterpret_cast<const char *>(A.s1lo) - q0;
+        s_off[8] = reinterpret_cast<const char *>(A.s2lo) - q0;
+        s_off[9] = reinterpret_cast<const char *>(A.qoutlo) - q0;
+        s_off[10] = reinterpret_cast<const char *>(A.accoutlo) - q0;
+        s_flags = (P.write_q ? F_Q : 0) | (P.use_ra ? F_RA : 0) | (P.use_rb ? F_RB : 0) |
+                  (P.write_acc ? F_ACC : 0) | (P.write_r ? F_R : 0) | (P.sponge ? F_SPONGE : 0) |
+                  (P.want_visc_max ? F_VISC : 0) | (Off<R>::on && P.comp ? F_COMP : 0) |
+                  (Off<R>::on && P.comp && P.write_qlo ? F_QLO : 0);
     }
-
-    const int bb = remap_block(blockIdx.x, gridDim.x, P.xcd_remap);
-    const int tile_x = bb % P.ntx;
-    const int rest = bb / P.ntx;
-    const int tile_y = rest % P.nty;
-    const int chunk = rest / P.nty;
-    const int x0 = tile_x * TX, y0 = tile_y * TY;
-    const int z0 = chunk * P.zc;
-    const int z1 = min(z0 + P.zc, P.mz);
-    const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
-    const int i = x0 + tx, j = y0 + ty;
-    const bool act = (i < P.mx) && (j < P.my);
-    const int cc = (ty + H) * LX + (tx + H);
-
-    double xpi = 1.0, dxvi = 1.0;
-    double vs[2 * SV + 1];
-#pragma unroll
-    for (int it = 0; it < 2 * SV + 1; it++) vs[it] = 0.0;
-    if (act) {
-        dxvi = P.dxv[i];
-        if (NONUNI) {
-            xpi = P.xp[i];
-#pragma unroll
-            for (int it = 0; it < 2 * SV + 1; it++) vs[it] = P.vsx[it * P.mx + i];
-        }
-    }
-
-    R win[NZ][W];
-#pragma unroll
-    for (int q = 0; q < NZ; q++)
-#pragma unroll
-        for (int a = 0; a < W; a++) win[q][a] = R(0);
-    auto push = [&](const R *t, int a) {
-        win[ZR][a] = t[QR];
-        win[ZU][a] = t[QU];
-        win[ZV][a] = t[QV];
-        win[ZW][a] = t[QW];
-        win[ZT][a] = t[QT];
-        win[ZD][a] = t[QD];
-        if (!UNITV) win[NZ - 1][a] = t[QM];
+    // array of the update at byte offset o from the stage input, plane poff
+    auto arr = [&](int which, long poff) -> R * {
+        const long long o = uniform64(s_off[which]);
+        return reinterpret_cast<R *>(const_cast<char *>(reinterpret_cast<const char *>(A.qin)) + o) + poff;
     };
-    const R cpR = R(P.cpR), invRe = R(P.invRe);
-    if (act) {
-#pragma unroll
-        for (int a = 0; a < W - 1; a++) {
-            R t[NQ];
-            load_prim<R, WALLX>(P, A, i, j, wrapi(z0 - H + a, P.mz), t);
-            push(t, a);
-        }
-    }
-    double vmax = 0.0;
-    const R dt = P.write_q ? R(*P.dt) : R(0);
-    const R dpdz = P.forcing ? R(*P.dpdz) : R(0);
 
-    for (int k = z0; k < z1; ++k) {
-        if (act) {
-            R t[NQ];
-            load_prim<R, WALLX>(P, A, i, j, wrapi(k + H, P.mz), t);
-            push(t, W - 1);
-        }
-        // z-window accessor; h and (for mu = T/Re) mu are rebuilt on the fly
-        auto FZ = [&](int q, int a) -> R {
-            switch (q) {
-            case QR: return win[ZR][H + a];
-            case QU: return win[ZU][H + a];
-            case QV: return win[ZV][H + a];
-            case QW: return win[ZW][H + a];
-            case QT: return win[ZT][H + a];
-            case QD: return win[ZD][H + a];
-            case QM: return UNITV ? win[ZT][H + a] * invRe : win[NZ - 1][H + a];
-            default: {
-                const R u = win[ZU][H + a], v = win[ZV][H + a], w = win[ZW][H + a];
-                return win[ZT][H + a] * cpR + R(0.5) * (u * u + v * v + w * w);
-            }
-            }
-        };
-        __syncthreads();
-        if (act) {
-#pragma unroll
-            for (int q = 0; q < NQ; q++) sq[q][cc] = FZ(q, 0);
-        } else {
-            R t[NQ];
-            load_prim<R, WALLX>(P, A, i, j, k, t);
-#pragma unroll
-            for (int q = 0; q < NQ; q++) sq[q][cc] = t[q];
-        }
-        for (int s = tid; s < NH; s += NT) {
-            int lx, ly;
-            if (s < 2 * H * TY) {
-                ly = H + s / (2 * H);
-                const int c2 = s % (2 * H);
-                lx = c2 < H ? c2 : TX + c2;
-            } else {
-                const int s2 = s - 2 * H * TY;
-                lx = H + s2 % TX;
-                const int rr = s2 / TX;
-                ly = rr < H ? rr : TY + rr;
-            }
-            R t[NQ];
-            load_prim<R, WALLX>(P, A, x0 + lx - H, y0 + ly - H, k, t);
-#pragma unroll
-            for (int q = 0; q < NQ; q++) sq[q][ly * LX + lx] = t[q];
-        }
-        __syncthreads();
-
-        if (act) {
-            auto FX = [&](int q, int a) -> R { return sq[q][cc + a]; };
-            auto FY = [&](int q, int a) -> R { return sq[q][cc + a * LX]; };
-            // scheduling barriers between the phases keep the compiler from
-            // hoisting every LDS read of the point to the top (register pressure)
-            R d1x[3], d2x[3], d1y[3], d2y[3], d1z[3], d2z[3];
-            vel_stencils<R, SV, 0, NONUNI>(P, FX, vs, xpi, d1x, d2x);
-            __builtin_amdgcn_sched_barrier(0);
-            vel_stencils<R, SV, 1, false>(P, FY, vs, xpi, d1y, d2y);
-            __builtin_amdgcn_sched_barrier(0);
-            vel_stencils<R, SV, 2, false>(P, FZ, vs, xpi, d1z, d2z);
-            __builtin_amdgcn_sched_barrier(0);
-            const R g[9] = {d1x[0], d1x[1], d1x[2], d1y[0], d1y[1], d1y[2], d1z[0], d1z[1], d1z[2]};
-            // x part (=), y part (+=), z part (+=) with the forcing, as
-            // src/cuda_rhs.cu:123-127,255-259,389-393 accumulate them
-            R rhs[5], part[5];
-            dir_terms<R, S, SV, 0, NONUNI>(P, FX, vs, xpi, g, d2x, rhs);
-            __builtin_amdgcn_sched_barrier(0);
-            dir_terms<R, S, SV, 1, false>(P, FY, vs, xpi, g, d2y, part);
-#pragma unroll
-            for (int q = 0; q < 5; q++) rhs[q] = rhs[q] + part[q];
-            __builtin_amdgcn_sched_barrier(0);
-            dir_terms<R, S, SV, 2, false>(P, FZ, vs, xpi, g, d2z, part);
-            __builtin_amdgcn_sched_barrier(0);
-            const R wc = win[ZW][H];
-            rhs[0] = rhs[0] + part[0];
-            rhs[1] = rhs[1] + part[1];
-            rhs[2] = rhs[2] + part[2];
-            rhs[3] = rhs[3] + (part[3] + dpdz);
-            rhs[4] = rhs[4] + (part[4] + dpdz * wc);
-
-            const long gidx = i + (long)P.mx * (j + (long)P.my * k);
-            const long fs = A.fs;
-            const int fl = __builtin_amdgcn_readfirstlane(s_flags);
-            R qc[5];
-#pragma unroll
-            for (int q = 0; q < 5; q++) qc[q] = A.qin[gidx + q * fs];
-            if (fl & 128) {
-                const R sg = R(P.spx[i] + P.spz[k]);
-                const long i2 = i + (long)P.mx * k, rs = (long)P.mx * P.mz;
-#pragma unroll
-                for (int q = 0; q < 5; q++) rhs[q] += sg * (R(P.ref[i2 + q * rs]) - qc[q]);
-            }
-            if (fl & 256) {
-                const double mu = (double)FZ(QM, 0);
-                const double v1 = mu / (dxvi * dxvi), v2 = mu * P.d_d2y, v3 = mu * P.d_d2z;
-                const double vm = v1 >= (v2 >= v3 ? v2 : v3) ? v1 : (v2 >= v3 ? v2 : v3);
-                vmax = vmax >= vm ? vmax : vm;
-            }
-            // stage update, one linear combination per field:
-            //   q_out = w_qc q_in + w_q0 q0 + w_acc acc + dt (w_ra ra + w_rb rb + w_r rhs)
-            R *const qout = (R *)s_ptr[0];
-            R *const rout = (R *)s_ptr[1];
-            const R *const ra = (const R *)s_ptr[2];
-            const R *const rb = (const R *)s_ptr[3];
-            const R *const q0 = (const R *)s_ptr[4];
-            R *const acc = (R *)s_ptr[5];
-#pragma unroll
-            for (int q = 0; q < 5; q++) {
-                const long gq = gidx + q * fs;
-                R base = R(s_w[0]) * qc[q];
-                if (fl & 1) base += R(s_w[1]) * q0[gq];
-                R acc_old = R(0);
-                if (fl & 2) {
-                    acc_old = acc[gq];
-                    base += R(s_w[2]) * acc_old;
-                }
-                R inc = R(0);
-                if (fl & 4) inc += R(s_w[3]) * ra[gq];
-                if (fl & 8) inc += R(s_w[4]) * rb[gq];
-                inc += R(s_w[5]) * rhs[q];
-                if (fl & 16) qout[gq] = base + dt * inc;
-                if (fl & 32) rout[gq] = rhs[q];
-                if (fl & 64) acc[gq] = R(s_w[6]) * qc[q] + R(s_w[7]) * acc_old + dt * (R(s_w[8]) * rhs[q]);
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < NZ; q++)
-#pragma unroll
-            for (int a = 0; a < W - 1; a++) win[q][a] = win[q][a + 1];
-    }
-    if (P.want_visc_max) block_max_atomic(vmax, P.visc_max_out, red);
-}
-
-// Dilatation of the stage input: du/dx + dv/dy + dw/dz at viscous order
-// (src/calc_stress.cu:20-63, 87-96), same tiling as k_rhs with halo SV.
-template <typename R, int SV, int TX, int TY, bool WALLX, bool NONUNI>
-__global__ __launch_bounds__(TX *TY) void k_dil(Params P, StagePtrs<R> A, R *dil) {
-    constexpr int H = SV, LX = TX + 2 * H, LY = TY + 2 * H, NT = TX * TY, W = 2 * H + 1;
-    constexpr int NH = 2 * H * TY + 2 * H * TX;
-    __shared__ R su[LY * LX], sv[LY * LX];
     const int bb = remap_block(blockIdx.x, gridDim.x, P.xcd_remap);
     const int tile_x = bb % P.ntx;
     const int rest = bb / P.ntx;
@@ -461,84 +369,385 @@ __global__ __launch_bounds__(TX *TY) void k_dil(Params P, StagePtrs<R> A, R *dil
     const bool act = (i < P.mx) && (j < P.my);
     const int cc = (ty + H) * LX + (tx + H);
     const long mxy = (long)P.mx * P.my;
-    double xpi = 1.0;
-    if (NONUNI && act) xpi = P.xp[i];
+    const long fs = A.fs;
 
-    auto uv_at = [&](int gi, int gj, int k, R &u, R &v) {
-        gj = wrapi(gj, P.my);
-        bool ghost = false;
-        if (WALLX) {
-            if (gi < 0) {
-                ghost = true;
-                gi = -1 - gi;
-            } else if (gi >= P.mx) {
-                ghost = true;
-                gi = 2 * P.mx - 1 - gi;
-            }
-            gi = gi < 0 ? 0 : (gi >= P.mx ? P.mx - 1 : gi);
-        } else {
-            gi = wrapi(gi, P.mx);
-        }
-        const long g = gi + P.mx * (long)gj + mxy * k;
-        const R inv = R(1) / A.qin[g];
-        u = A.qin[g + A.fs] * inv;
-        v = A.qin[g + 2 * A.fs] * inv;
-        if (ghost) {
-            u = -u;
-            v = -v;
-        }
-    };
-    auto w_at = [&](int k) -> R {
-        const long g = i + P.mx * (long)j + mxy * k;
-        return A.qin[g + 3 * A.fs] / A.qin[g];
-    };
+    // own column (lanes past the domain edge stage the ghost of their column)
+    int cside;
+    const unsigned coff = src_offset<WALLX>(P, i, j, ESZ, cside);
+    // this lane's halo point: x-halo columns first, then y-halo rows
+    const bool hact = tid < NH;
+    int hlx, hly;
+    if (tid < 2 * H * TY) {
+        hly = H + tid / (2 * H);
+        const int c2 = tid % (2 * H);
+        hlx = c2 < H ? c2 : TX + c2;
+    } else {
+        const int s2 = hact ? tid - 2 * H * TY : 0;
+        hlx = H + s2 % TX;
+        const int rr = s2 / TX;
+        hly = rr < H ? rr : TY + rr;
+    }
+    int hside;
+    const unsigned hoff = src_offset<WALLX>(P, x0 + hlx - H, y0 + hly - H, ESZ, hside);
+    const int hslot = hly * LX + hlx;
 
-    R ww[W];
+    double xpi = 1.0, inv_dx2 = 1.0;
+    double vs[2 * SV + 1];
+#pragma unroll
+    for (int it = 0; it < 2 * SV + 1; it++) vs[it] = 0.0;
+    if (act) {
+        const double dxv = P.dxv[i];
+        inv_dx2 = 1.0 / (dxv * dxv);
+        if (NONUNI) {
+            xpi = P.xp[i];
+#pragma unroll
+            for (int it = 0; it < 2 * SV + 1; it++) vs[it] = P.vsx[it * P.mx + i];
+        }
+    }
+
+    R win[NL][W];
+#pragma unroll
+    for (int q = 0; q < NL; q++)
+#pragma unroll
+        for (int a = 0; a < W; a++) win[q][a] = R(0);
+    auto push = [&](const R *t, int a) {
+#pragma unroll
+        for (int q = 0; q < NL; q++) win[q][a] = t[q];
+    };
+    const R invRe = R(P.invRe);
+    const R T0 = Off<R>::on ? R(P.T0) : R(0);  // T = T0 + staged T on offset storage
     if (act) {
 #pragma unroll
-        for (int a = 0; a < W - 1; a++) ww[a] = w_at(wrapi(z0 - H + a, P.mz));
-    }
-    for (int k = z0; k < z1; ++k) {
-        if (act) ww[W - 1] = w_at(wrapi(k + H, P.mz));
-        __syncthreads();
-        {
-            R u, v;
-            uv_at(i, j, k, u, v);
-            su[cc] = u;
-            sv[cc] = v;
+        for (int a = 0; a < W - 1; a++) {
+            Raw<R> x;
+            load_raw(A, (long)wrapi(z0 - H + a, P.mz) * mxy, coff, x);
+            R t[NQ];
+            to_prim<R, WALLX, UNITV>(P, x, 0, t);
+            push(t, a);
         }
-        for (int s = tid; s < NH; s += NT) {
-            int lx, ly;
-            if (s < 2 * H * TY) {
-                ly = H + s / (2 * H);
-                const int c2 = s % (2 * H);
-                lx = c2 < H ? c2 : TX + c2;
-            } else {
-                const int s2 = s - 2 * H * TY;
-                lx = H + s2 % TX;
-                const int rr = s2 / TX;
-                ly = rr < H ? rr : TY + rr;
+    }
+    Raw<R> zr, hr;  // prefetched: window feed (plane k + H) and halo point (plane k)
+    if (act) load_raw(A, (long)wrapi(z0 + H, P.mz) * mxy, coff, zr);
+    if (hact) load_raw(A, (long)z0 * mxy, hoff, hr);
+    double vmax = 0.0;
+    const double dtd = (P.write_q || P.write_acc) ? *P.dt : 0.0;
+    const R dt = R(dtd);
+    const R dpdz = P.forcing ? R(*P.dpdz) : R(0);
+    __syncthreads();  // update parameters in LDS
+    const int fl = __builtin_amdgcn_readfirstlane(s_flags);
+
+    for (int k = z0; k < z1; ++k) {
+        const int b = (k - z0) & 1;
+        const long poff = (long)k * mxy;
+        // operands of this plane's update
+        R e1[5], ea[5], eb[5], e2[5], e1l[5], e2l[5];
+#pragma unroll
+        for (int q = 0; q < 5; q++) e1[q] = ea[q] = eb[q] = e2[q] = e1l[q] = e2l[q] = R(0);
+        if (act) {
+            if (fl & F_Q) {
+                const R *p = arr(0, poff);
+#pragma unroll
+                for (int q = 0; q < 5; q++) e1[q] = ld_at(p + q * fs, coff);
             }
-            R u, v;
-            uv_at(x0 + lx - H, y0 + ly - H, k, u, v);
-            su[ly * LX + lx] = u;
-            sv[ly * LX + lx] = v;
+            if (fl & F_RA) {
+                const R *p = arr(1, poff);
+#pragma unroll
+                for (int q = 0; q < 5; q++) ea[q] = ld_at(p + q * fs, coff);
+            }
+            if (fl & F_RB) {
+                const R *p = arr(2, poff);
+#pragma unroll
+                for (int q = 0; q < 5; q++) eb[q] = ld_at(p + q * fs, coff);
+            }
+            if (fl & F_ACC) {
+                const R *p = arr(3, poff);
+#pragma unroll
+                for (int q = 0; q < 5; q++) e2[q] = ld_at(p + q * fs, coff);
+            }
+            if (Off<R>::on && (fl & F_COMP)) {
+                if (fl & F_Q) {
+                    const R *p = arr(7, poff);
+#pragma unroll
+                    for (int q = 0; q < 5; q++) e1l[q] = ld_at(p + q * fs, coff);
+                }
+                if (fl & F_ACC) {
+                    const R *p = arr(8, poff);
+#pragma unroll
+                    for (int q = 0; q < 5; q++) e2l[q] = ld_at(p + q * fs, coff);
+                }
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        if (act) {
+            R t[NQ];
+            to_prim<R, WALLX, UNITV>(P, zr, 0, t);
+            push(t, W - 1);
+#pragma unroll
+            for (int q = 0; q < NL; q++) sq[b][q][cc] = win[q][H];
+            if (k + 1 < z1) load_raw(A, (long)wrapi(k + 1 + H, P.mz) * mxy, coff, zr);
+        } else {
+            Raw<R> x;
+            load_raw(A, poff, coff, x);
+            R t[NQ];
+            to_prim<R, WALLX, UNITV>(P, x, cside, t);
+#pragma unroll
+            for (int q = 0; q < NL; q++) sq[b][q][cc] = t[q];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        if (hact) {
+            R t[NQ];
+            to_prim<R, WALLX, UNITV>(P, hr, hside, t);
+#pragma unroll
+            for (int q = 0; q < NL; q++) sq[b][q][hslot] = t[q];
+            if (k + 1 < z1) load_raw(A, poff + mxy, hoff, hr);
+        }
+        __syncthreads();
+
+        if (act) {
+            auto FX = [&](int q, int a) -> R {
+                return (UNITV && q == QM) ? (Off<R>::on ? T0 + sq[b][QT][cc + a] : sq[b][QT][cc + a]) * invRe
+                                          : sq[b][q][cc + a];
+            };
+            auto FY = [&](int q, int a) -> R {
+                return (UNITV && q == QM)
+                           ? (Off<R>::on ? T0 + sq[b][QT][cc + a * LX] : sq[b][QT][cc + a * LX]) * invRe
+                           : sq[b][q][cc + a * LX];
+            };
+            auto FZ = [&](int q, int a) -> R {
+                return (UNITV && q == QM) ? (Off<R>::on ? T0 + win[QT][H + a] : win[QT][H + a]) * invRe
+                                          : win[q][H + a];
+            };
+            // scheduling barriers between the phases keep the compiler from
+            // hoisting every LDS read of the point to the top (register pressure)
+            R d1x[3], d2x[3], d1y[3], d2y[3], d1z[3], d2z[3];
+            vel_stencils<R, SV, 0, NONUNI>(P, FX, vs, xpi, d1x, d2x);
+            __builtin_amdgcn_sched_barrier(0);
+            vel_stencils<R, SV, 1, false>(P, FY, vs, xpi, d1y, d2y);
+            __builtin_amdgcn_sched_barrier(0);
+            vel_stencils<R, SV, 2, false>(P, FZ, vs, xpi, d1z, d2z);
+            __builtin_amdgcn_sched_barrier(0);
+            const R g[9] = {d1x[0], d1x[1], d1x[2], d1y[0], d1y[1], d1y[2], d1z[0], d1z[1], d1z[2]};
+            // x part (=), y part (+=), z part (+=) with the forcing, as
+            // src/cuda_rhs.cu:123-127,255-259,389-393 accumulate them
+            R rhs[5], part[5];
+            dir_terms<R, S, SV, 0, NONUNI, UNITV>(P, FX, vs, xpi, g, d2x, rhs);
+            __builtin_amdgcn_sched_barrier(0);
+            dir_terms<R, S, SV, 1, false, UNITV>(P, FY, vs, xpi, g, d2y, part);
+#pragma unroll
+            for (int q = 0; q < 5; q++) rhs[q] = rhs[q] + part[q];
+            __builtin_amdgcn_sched_barrier(0);
+            dir_terms<R, S, SV, 2, false, UNITV>(P, FZ, vs, xpi, g, d2z, part);
+            __builtin_amdgcn_sched_barrier(0);
+            const R wc = win[QW][H];
+            rhs[0] = rhs[0] + part[0];
+            rhs[1] = rhs[1] + part[1];
+            rhs[2] = rhs[2] + part[2];
+            rhs[3] = rhs[3] + (part[3] + dpdz);
+            rhs[4] = rhs[4] + (part[4] + dpdz * wc);
+
+            if (fl & F_SPONGE) {
+                const long i2 = i + (long)P.mx * k, rs = (long)P.mx * P.mz;
+                const R *qc = A.qin + poff;
+                if constexpr (Off<R>::on) {
+                    const double sg = P.spx[i] + P.spz[k];
+                    const double off[5] = {P.rho0, 0.0, 0.0, 0.0, P.E0};
+#pragma unroll
+                    for (int q = 0; q < 5; q++)
+                        rhs[q] += R(sg * (P.ref[i2 + q * rs] - ((double)ld_at(qc + q * fs, coff) + off[q])));
+                } else {
+                    const R sg = R(P.spx[i] + P.spz[k]);
+#pragma unroll
+                    for (int q = 0; q < 5; q++) rhs[q] += sg * (R(P.ref[i2 + q * rs]) - ld_at(qc + q * fs, coff));
+                }
+            }
+            if (fl & F_VISC) {
+                const double mu = (double)FZ(QM, 0);
+                const double v1 = mu * inv_dx2, v2 = mu * P.d_d2y, v3 = mu * P.d_d2z;
+                const double vm = v1 >= (v2 >= v3 ? v2 : v3) ? v1 : (v2 >= v3 ? v2 : v3);
+                vmax = vmax >= vm ? vmax : vm;
+            }
+            if constexpr (Off<R>::on) {
+                // FP64 update of the compensated pair, rounded to hi + lo
+                const bool comp = fl & F_COMP;
+                if (fl & F_Q) {
+                    R *const qo = arr(4, poff);
+                    const bool wlo = fl & F_QLO;
+                    R *const qol = wlo ? arr(9, poff) : qo;
+#pragma unroll
+                    for (int q = 0; q < 5; q++) {
+                        double inc = 0.0;
+                        if (fl & F_RA) inc += s_w[0] * (double)ea[q];
+                        if (fl & F_RB) inc += s_w[1] * (double)eb[q];
+                        inc += s_w[2] * (double)rhs[q];
+                        const double x = ((double)e1[q] + (double)e1l[q]) + dtd * inc;
+                        const R hi = R(x);
+                        st_at(qo + q * fs, coff, hi);
+                        if (wlo) st_at(qol + q * fs, coff, R(x - (double)hi));
+                    }
+                }
+                if (fl & F_R) {
+                    R *const ro = arr(5, poff);
+#pragma unroll
+                    for (int q = 0; q < 5; q++) st_at(ro + q * fs, coff, rhs[q]);
+                }
+                if (fl & F_ACC) {
+                    R *const ao = arr(6, poff);
+                    R *const aol = comp ? arr(10, poff) : ao;
+#pragma unroll
+                    for (int q = 0; q < 5; q++) {
+                        const double x = ((double)e2[q] + (double)e2l[q]) + dtd * (s_w[3] * (double)rhs[q]);
+                        const R hi = R(x);
+                        st_at(ao + q * fs, coff, hi);
+                        if (comp) st_at(aol + q * fs, coff, R(x - (double)hi));
+                    }
+                }
+            } else {
+                if (fl & F_Q) {
+                    R *const qo = arr(4, poff);
+#pragma unroll
+                    for (int q = 0; q < 5; q++) {
+                        R inc = R(0);
+                        if (fl & F_RA) inc += R(s_w[0]) * ea[q];
+                        if (fl & F_RB) inc += R(s_w[1]) * eb[q];
+                        inc += R(s_w[2]) * rhs[q];
+                        st_at(qo + q * fs, coff, e1[q] + dt * inc);
+                    }
+                }
+                if (fl & F_R) {
+                    R *const ro = arr(5, poff);
+#pragma unroll
+                    for (int q = 0; q < 5; q++) st_at(ro + q * fs, coff, rhs[q]);
+                }
+                if (fl & F_ACC) {
+                    R *const ao = arr(6, poff);
+#pragma unroll
+                    for (int q = 0; q < 5; q++) st_at(ao + q * fs, coff, e2[q] + dt * (R(s_w[3]) * rhs[q]));
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < NL; q++)
+#pragma unroll
+            for (int a = 0; a < W - 1; a++) win[q][a] = win[q][a + 1];
+    }
+    if (P.want_visc_max) block_max_atomic(vmax, P.visc_max_out, red);
+}
+
+// Dilatation of the stage input: du/dx + dv/dy + dw/dz at viscous order
+// (src/calc_stress.cu:20-63, 87-96).  Same tiling and prefetch scheme as
+// k_rhs with halo SV: u and v of plane k (tile + halo) in double-buffered
+// LDS, w of the own column in a register window.
+template <typename R, int SV, int TX, int TY, bool WALLX, bool NONUNI>
+__global__ __launch_bounds__(TX *TY) void k_dil(Params P, StagePtrs<R> A, R *dil) {
+    constexpr int H = SV, LX = TX + 2 * H, LY = TY + 2 * H, NT = TX * TY, W = 2 * H + 1;
+    constexpr int NH = 2 * H * TY + 2 * H * TX;
+    static_assert(NH <= NT, "one halo point per lane");
+    constexpr unsigned ESZ = sizeof(R);
+    __shared__ R su[2][LY * LX], sv[2][LY * LX];
+    const int bb = remap_block(blockIdx.x, gridDim.x, P.xcd_remap);
+    const int tile_x = bb % P.ntx;
+    const int rest = bb / P.ntx;
+    const int tile_y = rest % P.nty;
+    const int chunk = rest / P.nty;
+    const int x0 = tile_x * TX, y0 = tile_y * TY;
+    const int z0 = chunk * P.zc;
+    const int z1 = min(z0 + P.zc, P.mz);
+    const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
+    const int i = x0 + tx, j = y0 + ty;
+    const bool act = (i < P.mx) && (j < P.my);
+    const int cc = (ty + H) * LX + (tx + H);
+    const long mxy = (long)P.mx * P.my;
+    const long fs = A.fs;
+
+    int cside;
+    const unsigned coff = src_offset<WALLX>(P, i, j, ESZ, cside);
+    const bool hact = tid < NH;
+    int hlx, hly;
+    if (tid < 2 * H * TY) {
+        hly = H + tid / (2 * H);
+        const int c2 = tid % (2 * H);
+        hlx = c2 < H ? c2 : TX + c2;
+    } else {
+        const int s2 = hact ? tid - 2 * H * TY : 0;
+        hlx = H + s2 % TX;
+        const int rr = s2 / TX;
+        hly = rr < H ? rr : TY + rr;
+    }
+    int hside;
+    const unsigned hoff = src_offset<WALLX>(P, x0 + hlx - H, y0 + hly - H, ESZ, hside);
+    const int hslot = hly * LX + hlx;
+    // velocities are antisymmetric across an isothermal no-slip wall
+    const R csg = cside ? R(-1) : R(1), hsg = hside ? R(-1) : R(1);
+    double xpi = 1.0;
+    if (NONUNI && act) xpi = P.xp[i];
+    const R r0 = Off<R>::on ? R(P.rho0) : R(0);  // rho = rho0 + stored value on offset storage
+    auto rho = [&](R stored) -> R { return Off<R>::on ? r0 + stored : stored; };
+
+    // u, v of the own column / halo point at plane poff, w of the own column
+    auto ld3 = [&](long poff, unsigned o, R v3[3]) {
+        const R *b = A.qin + poff;
+        v3[0] = ld_at(b, o);
+        v3[1] = ld_at(b + fs, o);
+        v3[2] = ld_at(b + 2 * fs, o);
+    };
+    auto ldw = [&](long poff, R v2[2]) {
+        const R *b = A.qin + poff;
+        v2[0] = ld_at(b, coff);
+        v2[1] = ld_at(b + 3 * fs, coff);
+    };
+    R ww[W];
+#pragma unroll
+    for (int a = 0; a < W; a++) ww[a] = R(0);
+    if (act) {
+#pragma unroll
+        for (int a = 0; a < W - 1; a++) {
+            R t[2];
+            ldw((long)wrapi(z0 - H + a, P.mz) * mxy, t);
+            ww[a] = t[1] / rho(t[0]);
+        }
+    }
+    R zr[2] = {R(1) - r0, R(0)}, cr[3], hr[3] = {R(1) - r0, R(0), R(0)};
+    if (act) ldw((long)wrapi(z0 + H, P.mz) * mxy, zr);
+    ld3((long)z0 * mxy, coff, cr);
+    if (hact) ld3((long)z0 * mxy, hoff, hr);
+
+    for (int k = z0; k < z1; ++k) {
+        const int b = (k - z0) & 1;
+        const long poff = (long)k * mxy;
+        const bool more = k + 1 < z1;
+        ww[W - 1] = zr[1] / rho(zr[0]);
+        if (act && more) ldw((long)wrapi(k + 1 + H, P.mz) * mxy, zr);
+        {
+            const R inv = R(1) / rho(cr[0]);
+            su[b][cc] = csg * (cr[1] * inv);
+            sv[b][cc] = csg * (cr[2] * inv);
+        }
+        if (more) ld3(poff + mxy, coff, cr);
+        if (hact) {
+            const R inv = R(1) / rho(hr[0]);
+            su[b][hslot] = hsg * (hr[1] * inv);
+            sv[b][hslot] = hsg * (hr[2] * inv);
+            if (more) ld3(poff + mxy, hoff, hr);
         }
         __syncthreads();
         if (act) {
             R a1 = R(0), a2 = R(0), a3 = R(0);
 #pragma unroll
-            for (int k = SV; k >= 1; k--) {
-                const R c = R(FD<SV>::a(k));
-                a1 += c * (su[cc + k] - su[cc - k]);
-                a2 += c * (sv[cc + k * LX] - sv[cc - k * LX]);
-                a3 += c * (ww[H + k] - ww[H - k]);
+            for (int kk = SV; kk >= 1; kk--) {
+                const R c = R(FD<SV>::a(kk));
+                a1 += c * (su[b][cc + kk] - su[b][cc - kk]);
+                a2 += c * (sv[b][cc + kk * LX] - sv[b][cc - kk * LX]);
+                a3 += c * (ww[H + kk] - ww[H - kk]);
             }
-            a1 = a1 * R(P.d_dx);
-            if (NONUNI) a1 = a1 * R(xpi);
+            if (Off<R>::on && NONUNI) {
+                a1 = R((double)a1 * (P.d_dx * xpi));  // metric in FP64
+            } else {
+                a1 = a1 * R(P.d_dx);
+                if (NONUNI) a1 = a1 * R(xpi);
+            }
             a2 = a2 * R(P.d_dy);
             a3 = a3 * R(P.d_dz);
-            dil[i + P.mx * (long)j + mxy * k] = a1 + a2 + a3;
+            st_at(dil + poff, coff, a1 + a2 + a3);
         }
 #pragma unroll
         for (int a = 0; a < W - 1; a++) ww[a] = ww[a + 1];

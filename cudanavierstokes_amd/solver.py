@@ -39,7 +39,8 @@ class Config:
     check_cfl: int = 10
     check_bulk: int = 10
     fix_y_work: int = 0
-    precision: int = 0
+    precision: int = 0  # 0 FP64 storage, 1 FP32 storage (offsets from a reference state)
+    compensated: int = 1  # FP32 storage: compensated (hi + lo) RK sums
     device: int = 0
     Lx: float = 2 * np.pi
     Ly: float = 2 * np.pi

@@ -59,6 +59,7 @@ typedef struct cns_config {
     int check_bulk;       /* checkBulk */
     int fix_y_work;       /* 0: y viscous work exactly as src/cuda_rhs.cu:175; 1: corrected */
     int precision;        /* CNS_FP64 or CNS_FP32 */
+    int compensated;      /* CNS_FP32: carry a compensation (low) part of the state through the RK sums */
     int device;           /* HIP device ordinal */
     double len_x, len_y, len_z;  /* Lx, Ly, Lz */
     double reynolds, prandtl, mach, gamma; /* Re, Pr, Ma, gam */
