@@ -43,6 +43,8 @@ EXPORTS = [
     "cns_fill_taylor_green", "cns_fill_channel", "cns_set_state", "cns_get_state", "cns_rhs",
     "cns_step", "cns_set_sponge", "cns_get_scalars", "cns_sync", "cns_profile_enable",
     "cns_profile_read", "cns_time_steps", "cns_device_bytes",
+    "cns_batch_create", "cns_batch_tune", "cns_batch_step", "cns_batch_time_steps", "cns_batch_group_info",
+    "cns_batch_groups", "cns_batch_destroy", "cns_batch_last_error",
 ]
 
 _lib = None
@@ -82,8 +84,18 @@ def load(path=None):
     L.cns_profile_read.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
     L.cns_time_steps.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
     L.cns_device_bytes.argtypes = [vp, C.POINTER(C.c_uint64)]
+    L.cns_batch_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(vp)]
+    L.cns_batch_tune.argtypes = [vp]
+    L.cns_batch_step.argtypes = [vp, C.c_int]
+    L.cns_batch_time_steps.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
+    ip = C.POINTER(C.c_int)
+    L.cns_batch_group_info.argtypes = [vp, C.c_int, ip, ip, ip, ip, ip]
+    L.cns_batch_groups.argtypes = [vp]
+    L.cns_batch_destroy.argtypes = [vp]
+    L.cns_batch_last_error.restype = C.c_char_p
+    L.cns_batch_last_error.argtypes = [vp]
     for name in EXPORTS:
-        if name != "cns_last_error":
+        if name not in ("cns_last_error", "cns_batch_last_error"):
             getattr(L, name).restype = C.c_int
     if path is None:
         _lib = L
@@ -93,4 +105,10 @@ def load(path=None):
 def check(rc, handle=None, what=""):
     if rc != CNS_OK:
         msg = load().cns_last_error(handle)
+        raise CnsError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+
+def check_batch(rc, handle=None, what=""):
+    if rc != CNS_OK:
+        msg = load().cns_batch_last_error(handle)
         raise CnsError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")

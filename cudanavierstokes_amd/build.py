@@ -35,9 +35,10 @@ def build(verbose=False, jobs=None):
     os.makedirs(BUILD, exist_ok=True)
     jobs = jobs or min(8, os.cpu_count() or 1)
     units = []
-    for s in (1, 2, 3, 4):
-        units.append((os.path.join(CSRC, "cns_kernels.hip"), os.path.join(BUILD, f"cns_kernels_s{s}.o"),
-                      [f"-DCNS_S={s}"]))
+    for s in (4, 3, 2, 1):
+        for r in ("double", "float"):
+            units.append((os.path.join(CSRC, "cns_kernels.hip"), os.path.join(BUILD, f"cns_kernels_s{s}_{r}.o"),
+                          [f"-DCNS_S={s}", f"-DCNS_R={r}"]))
     units.append((os.path.join(CSRC, "cns_api.hip"), os.path.join(BUILD, "cns_api.o"), []))
     cmds = [[HIPCC] + CFLAGS + extra + ["-c", src, "-o", obj] for src, obj, extra in units]
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:

@@ -12,6 +12,7 @@
 //   cns_destroy       clearSolver (src/cuda_utils.cu:521)
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -22,6 +23,7 @@
 
 #include "../../include/cns.h"
 #include "cns_aux.hpp"
+#include "cns_batch.hpp"
 #include "cns_launch.hpp"
 
 using namespace cns;
@@ -105,6 +107,9 @@ struct cns_solver {
     double *d_xp = nullptr, *d_vsx = nullptr, *d_dxv = nullptr, *d_spx = nullptr, *d_spz = nullptr;
     double *d_ref = nullptr;  // 5 sponge reference fields, mx*mz each
     Scalars *d_sc = nullptr;
+    StepCtl *d_ctl = nullptr;  // per-member step control when advanced in a batch
+    bool in_batch = false;
+    long bsteps = 0;  // steps taken in batches since the last set_state / cns_step (CFL phase)
     double *d_part = nullptr;
     // state storage: FP64, or FP32 offsets from (rho0, E0) with optional
     // compensation (low) parts
@@ -398,6 +403,7 @@ extern "C" int cns_destroy(cns_solver *s) {
     fr(s->d_spz);
     fr(s->d_ref);
     if (s->d_sc) (void)hipFree(s->d_sc);
+    if (s->d_ctl) (void)hipFree(s->d_ctl);
     fr(s->d_part);
     auto fv = [](void *p) {
         if (p) (void)hipFree(p);
@@ -481,6 +487,10 @@ extern "C" int cns_create(const cns_config *cfg, cns_solver **out) {
     }
     for (int a = 0; a < s->nrhs; a++)
         if ((rc = balloc(s, &s->rbuf[a], 5 * (size_t)n * s->esz))) return bail(rc);
+    if (hipMalloc((void **)&s->d_ctl, sizeof(StepCtl)) != hipSuccess) {
+        s->err = "hipMalloc(step control) failed";
+        return bail(CNS_ERR_NOMEM);
+    }
     if (hipMalloc((void **)&s->d_sc, sizeof(Scalars)) != hipSuccess) {
         s->err = "hipMalloc(scalars) failed";
         return bail(CNS_ERR_NOMEM);
@@ -628,6 +638,7 @@ static void set_offsets(cns_solver *s, double rho0, double E0) {
 extern "C" int cns_set_state(cns_solver *s, const double *r, const double *u, const double *v, const double *w,
                              const double *e) {
     if (!s || !r || !u || !v || !w || !e) return fail(s, CNS_ERR_ARG, "cns_set_state: null argument");
+    if (s->in_batch) return fail(s, CNS_ERR_STATE, "cns_set_state: the solver is advanced by a batch");
     HIPCHK(s, hipSetDevice(s->dev));
     const long n = s->n;
     s->cur = 0;
@@ -661,6 +672,7 @@ extern "C" int cns_set_state(cns_solver *s, const double *r, const double *u, co
         HIPCHK(s, hipMemcpy(s->sets[0], hi.data(), hi.size() * sizeof(float), hipMemcpyHostToDevice));
         if (s->comp) HIPCHK(s, hipMemcpy(s->lo[0], lo.data(), lo.size() * sizeof(float), hipMemcpyHostToDevice));
     }
+    s->bsteps = 0;
     // time restarts at zero (initDevice, src/cuda_utils.cu:386)
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(1), 0, s->stream, &s->d_sc->time, 1L, 0.0);
     LAUNCHED(s, "k_fill");
@@ -770,6 +782,7 @@ static int rhs_t(cns_solver *s, double *const dst[5]) {
 
 extern "C" int cns_rhs(cns_solver *s, double *rr, double *ru, double *rv, double *rw, double *re) {
     if (!s || !rr || !ru || !rv || !rw || !re) return fail(s, CNS_ERR_ARG, "cns_rhs: null argument");
+    if (s->in_batch) return fail(s, CNS_ERR_STATE, "cns_rhs: the solver is advanced by a batch");
     if (!s->have_state) return fail(s, CNS_ERR_STATE, "cns_rhs: no state set");
     HIPCHK(s, hipSetDevice(s->dev));
     double *const dst[5] = {rr, ru, rv, rw, re};
@@ -791,6 +804,89 @@ static Params stage_params(const cns_solver *s, double w_ra, double w_rb, double
 
 static const double SMR_ALPHA[3] = {0., -17. / 60., -5. / 12.};
 static const double SMR_BETA[3] = {8. / 15., 5. / 12., 3. / 4.};
+
+static int n_stages(const cns_solver *s) { return s->cfg.scheme == CNS_RK4 ? 4 : 3; }
+
+// set holding the state after a step that started from set cur (the
+// low-storage RK3 alternates between its two sets)
+static int next_cur(const cns_solver *s, int cur) { return s->cfg.scheme == CNS_RK3_LOWSTORAGE ? 1 - cur : cur; }
+
+// Parameters and arrays of stage st of a step that starts from set cur;
+// returns the set the stage reads with stencils.  last_visc: the last stage
+// accumulates the viscous CFL limit for the next check.
+template <typename R>
+static int stage_setup(const cns_solver *s, int cur, int st, bool last_visc, Params &P, StagePtrs<R> &A) {
+    const cns_config &c = s->cfg;
+    std::memset(&A, 0, sizeof(A));
+    int in = cur;
+    if (c.scheme == CNS_RK3_LOWSTORAGE) {
+        // q <- q + dt (alpha_s r_prev + beta_s r_new)  (sumLowStorageRK3)
+        const int a = cur, b = 1 - cur;
+        const int ins[3] = {a, b, a}, outs[3] = {b, a, b};
+        in = ins[st];
+        P = stage_params(s, st ? SMR_ALPHA[st] : 0, 0, SMR_BETA[st]);
+        P.write_r = st < 2;
+        P.want_visc_max = (st == 2 && last_visc) ? 1 : 0;
+        A.s1 = as<R>(s->sets[in]);
+        A.qout = as<R>(s->sets[outs[st]]);
+        A.rout = st < 2 ? as<R>(s->rbuf[st]) : nullptr;
+        A.ra = st == 0 ? nullptr : as<R>(s->rbuf[st - 1]);
+        // every stage output is the next stage's running sum
+        A.s1lo = as<R>(s->lo[in]);
+        A.qoutlo = as<R>(s->lo[outs[st]]);
+        P.write_qlo = 1;
+    } else if (c.scheme == CNS_RK3_KUTTA) {
+        // q1 = q0 + dt/2 k1; q2 = q0 + dt (2 k2 - k1); q = q0 + dt (k1 + 4 k2 + k3)/6
+        const int q0 = cur, s1 = (cur + 1) % 3, s2 = (cur + 2) % 3;
+        const int ins[3] = {q0, s1, s2}, outs[3] = {s1, s2, q0};
+        in = ins[st];
+        if (st == 0) P = stage_params(s, 0, 0, 0.5);
+        else if (st == 1) P = stage_params(s, -1.0, 0, 2.0);
+        else P = stage_params(s, 1.0 / 6.0, 4.0 / 6.0, 1.0 / 6.0);
+        P.write_r = st < 2;
+        P.want_visc_max = (st == 2 && last_visc) ? 1 : 0;
+        A.qout = as<R>(s->sets[outs[st]]);
+        A.s1 = as<R>(s->sets[q0]);  // the stage-0 input is q0 itself
+        A.rout = st < 2 ? as<R>(s->rbuf[st]) : nullptr;
+        A.ra = as<R>(s->rbuf[0]);
+        A.rb = as<R>(s->rbuf[1]);
+        // q1, q2 only feed the RHS; the step result carries a low part
+        A.s1lo = as<R>(s->lo[q0]);
+        A.qoutlo = as<R>(s->lo[outs[st]]);
+        P.write_qlo = st == 2;
+    } else {
+        // classic RK4 with an accumulator set: acc = q0 + dt/6 k1 + dt/3 k2 + dt/3 k3,
+        // q = acc + dt/6 k4.  Sets: A = current, B/D stage inputs, C accumulator.
+        const int sa = cur, sb = (cur + 1) % 4, scc = (cur + 2) % 4, sd = (cur + 3) % 4;
+        const int ins[4] = {sa, sb, sd, sb}, outs[4] = {sb, sd, sb, sa};
+        in = ins[st];
+        if (st == 0) P = stage_params(s, 0, 0, 0.5);
+        else if (st == 1) P = stage_params(s, 0, 0, 0.5);
+        else if (st == 2) P = stage_params(s, 0, 0, 1.0);
+        else P = stage_params(s, 0, 0, 1.0 / 6.0);
+        A.qout = as<R>(s->sets[outs[st]]);
+        A.s1 = as<R>(s->sets[st < 3 ? sa : scc]);
+        A.s1lo = as<R>(s->lo[st < 3 ? sa : scc]);
+        // stage inputs only feed the RHS; the step result carries a low part
+        A.qoutlo = as<R>(s->lo[outs[st]]);
+        P.write_qlo = st == 3;
+        if (st < 3) {
+            // acc = q0 + dt/6 k1, then acc += dt/3 k2, acc += dt/3 k3
+            P.write_acc = 1;
+            P.v_r = st == 0 ? 1.0 / 6.0 : 1.0 / 3.0;
+            A.s2 = as<R>(s->sets[st == 0 ? sa : scc]);
+            A.accout = as<R>(s->sets[scc]);
+            A.s2lo = as<R>(s->lo[st == 0 ? sa : scc]);
+            A.accoutlo = as<R>(s->lo[scc]);
+        }
+        P.want_visc_max = (st == 3 && last_visc) ? 1 : 0;
+    }
+    A.qin = as<R>(s->sets[in]);
+    A.fs = s->n;
+    A.dil = as<R>(s->d_dil);
+    fill_unused(A);
+    return in;
+}
 
 // queue nsteps steps on the stream; outputs stay on the device
 template <typename R>
@@ -835,83 +931,14 @@ static int queue_steps_t(cns_solver *s, int nsteps, bool want_par, bool want_tim
             LAUNCHED(s, "k_final");
         }
         const bool need_visc = ((istep + 1) % c.check_cfl == 0) || (istep == nsteps - 1);
-        int rc;
-        if (c.scheme == CNS_RK3_LOWSTORAGE) {
-            // q <- q + dt (alpha_s r_prev + beta_s r_new)  (sumLowStorageRK3)
-            const int a = s->cur, b = 1 - s->cur;
-            const int in[3] = {a, b, a}, outs[3] = {b, a, b};
-            for (int st = 0; st < 3; st++) {
-                Params P = stage_params(s, st ? SMR_ALPHA[st] : 0, 0, SMR_BETA[st]);
-                P.write_r = st < 2;
-                P.want_visc_max = (st == 2 && need_visc) ? 1 : 0;
-                StagePtrs<R> A{};
-                A.s1 = as<R>(s->sets[in[st]]);
-                A.qout = as<R>(s->sets[outs[st]]);
-                A.rout = st < 2 ? as<R>(s->rbuf[st]) : nullptr;
-                A.ra = st == 0 ? nullptr : as<R>(s->rbuf[st - 1]);
-                // every stage output is the next stage's running sum
-                A.s1lo = as<R>(s->lo[in[st]]);
-                A.qoutlo = as<R>(s->lo[outs[st]]);
-                P.write_qlo = 1;
-                if ((rc = run_stage<R>(s, in[st], P, A))) return rc;
-            }
-            s->cur = b;
-        } else if (c.scheme == CNS_RK3_KUTTA) {
-            // q1 = q0 + dt/2 k1; q2 = q0 + dt (2 k2 - k1); q = q0 + dt (k1 + 4 k2 + k3)/6
-            const int q0 = s->cur, s1 = (s->cur + 1) % 3, s2 = (s->cur + 2) % 3;
-            const int in[3] = {q0, s1, s2}, outs[3] = {s1, s2, q0};
-            for (int st = 0; st < 3; st++) {
-                Params P;
-                if (st == 0) P = stage_params(s, 0, 0, 0.5);
-                else if (st == 1) P = stage_params(s, -1.0, 0, 2.0);
-                else P = stage_params(s, 1.0 / 6.0, 4.0 / 6.0, 1.0 / 6.0);
-                P.write_r = st < 2;
-                P.want_visc_max = (st == 2 && need_visc) ? 1 : 0;
-                StagePtrs<R> A{};
-                A.qout = as<R>(s->sets[outs[st]]);
-                A.s1 = as<R>(s->sets[q0]);  // the stage-0 input is q0 itself
-                A.rout = st < 2 ? as<R>(s->rbuf[st]) : nullptr;
-                A.ra = as<R>(s->rbuf[0]);
-                A.rb = as<R>(s->rbuf[1]);
-                // q1, q2 only feed the RHS; the step result carries a low part
-                A.s1lo = as<R>(s->lo[q0]);
-                A.qoutlo = as<R>(s->lo[outs[st]]);
-                P.write_qlo = st == 2;
-                if ((rc = run_stage<R>(s, in[st], P, A))) return rc;
-            }
-            s->cur = q0;
-        } else {
-            // classic RK4 with an accumulator set: acc = q0 + dt/6 k1 + dt/3 k2 + dt/3 k3,
-            // q = acc + dt/6 k4.  Sets: A = current, B/D stage inputs, C accumulator.
-            const int sa = s->cur, sb = (s->cur + 1) % 4, scc = (s->cur + 2) % 4, sd = (s->cur + 3) % 4;
-            const int in[4] = {sa, sb, sd, sb}, outs[4] = {sb, sd, sb, sa};
-            for (int st = 0; st < 4; st++) {
-                Params P;
-                if (st == 0) P = stage_params(s, 0, 0, 0.5);
-                else if (st == 1) P = stage_params(s, 0, 0, 0.5);
-                else if (st == 2) P = stage_params(s, 0, 0, 1.0);
-                else P = stage_params(s, 0, 0, 1.0 / 6.0);
-                StagePtrs<R> A{};
-                A.qout = as<R>(s->sets[outs[st]]);
-                A.s1 = as<R>(s->sets[st < 3 ? sa : scc]);
-                A.s1lo = as<R>(s->lo[st < 3 ? sa : scc]);
-                // stage inputs only feed the RHS; the step result carries a low part
-                A.qoutlo = as<R>(s->lo[outs[st]]);
-                P.write_qlo = st == 3;
-                if (st < 3) {
-                    // acc = q0 + dt/6 k1, then acc += dt/3 k2, acc += dt/3 k3
-                    P.write_acc = 1;
-                    P.v_r = st == 0 ? 1.0 / 6.0 : 1.0 / 3.0;
-                    A.s2 = as<R>(s->sets[st == 0 ? sa : scc]);
-                    A.accout = as<R>(s->sets[scc]);
-                    A.s2lo = as<R>(s->lo[st == 0 ? sa : scc]);
-                    A.accoutlo = as<R>(s->lo[scc]);
-                }
-                P.want_visc_max = (st == 3 && need_visc) ? 1 : 0;
-                if ((rc = run_stage<R>(s, in[st], P, A))) return rc;
-            }
-            s->cur = sa;
+        for (int st = 0; st < n_stages(s); st++) {
+            Params P;
+            StagePtrs<R> A;
+            const int in = stage_setup<R>(s, s->cur, st, need_visc, P, A);
+            int rc;
+            if ((rc = run_stage<R>(s, in, P, A))) return rc;
         }
+        s->cur = next_cur(s, s->cur);
     }
     return CNS_OK;
 }
@@ -938,6 +965,8 @@ extern "C" int cns_step(cns_solver *s, int nsteps, double *kin, double *enst, do
     if (!s) return fail(nullptr, CNS_ERR_ARG, "null solver");
     if (nsteps < 0) return fail(s, CNS_ERR_ARG, "cns_step: nsteps < 0");
     if (!s->have_state) return fail(s, CNS_ERR_STATE, "cns_step: no state set");
+    if (s->in_batch) return fail(s, CNS_ERR_STATE, "cns_step: the solver is advanced by a batch");
+    s->bsteps = 0;
     if (nsteps == 0) return CNS_OK;
     HIPCHK(s, hipSetDevice(s->dev));
     const bool want_par = kin || enst;
@@ -961,6 +990,8 @@ extern "C" int cns_step(cns_solver *s, int nsteps, double *kin, double *enst, do
 extern "C" int cns_time_steps(cns_solver *s, int nsteps, double *ms) {
     if (!s || !ms) return fail(s, CNS_ERR_ARG, "cns_time_steps: null argument");
     if (!s->have_state) return fail(s, CNS_ERR_STATE, "cns_time_steps: no state set");
+    if (s->in_batch) return fail(s, CNS_ERR_STATE, "cns_time_steps: the solver is advanced by a batch");
+    s->bsteps = 0;
     HIPCHK(s, hipSetDevice(s->dev));
     hipEvent_t a, b;
     HIPCHK(s, hipEventCreate(&a));
@@ -1039,3 +1070,459 @@ extern "C" int cns_device_bytes(const cns_solver *s, uint64_t *bytes) {
     *bytes = s->dev_bytes;
     return CNS_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Batched advance (include/cns.h: cns_batch_*)
+// ---------------------------------------------------------------------------
+
+namespace {
+
+struct BatchGroup {
+    std::vector<cns_solver *> m;
+    bool fp32 = false, wallx = false, nonuni = false, unitv = false, forcing = false;
+    int S = 0, SV = 0, scheme = 0, nst = 0;
+    int pencil = 64, grid = 0, remap = 1, nwork = 0;
+    void *d_jobs = nullptr;  // nst arrays of m.size() jobs
+    size_t job_size = 0;
+};
+
+}  // namespace
+
+struct cns_batch {
+    std::string err;
+    int dev = 0, cus = 256;
+    hipStream_t stream = nullptr;
+    std::vector<cns_solver *> members;
+    std::vector<int> cur0;     // each member's state set when the batch was made
+    std::vector<long> phase0;  // each member's batch step count when the batch was made
+    std::vector<BatchGroup> groups;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    long steps = 0;
+    bool sync_check = false;
+};
+
+#define BCHK(b, expr)                                                                                     \
+    do {                                                                                                  \
+        hipError_t e_ = (expr);                                                                           \
+        if (e_ != hipSuccess) {                                                                           \
+            (b)->err = std::string(#expr) + ": " + hipGetErrorString(e_);                                 \
+            return CNS_ERR_HIP;                                                                           \
+        }                                                                                                 \
+    } while (0)
+
+static int bfail(cns_batch *b, int code, const std::string &msg) {
+    if (b)
+        b->err = msg;
+    else
+        g_err = msg;
+    return code;
+}
+
+// z chunks of member m under pencil length L (at least the window depth)
+static void member_geometry(const cns_solver *m, int L, int &zc, int &nzc, int &nwork) {
+    const int mz = m->cfg.nz;
+    const int lo = std::min(2 * m->S, mz);
+    zc = std::max(std::min(L, mz), lo);
+    nzc = (mz + zc - 1) / zc;
+    nwork = m->ntx * m->nty * nzc;
+}
+
+// Jobs of every stage of group g under pencil L.  scratch: the tuning form,
+// whose outputs all go to the member's first rhs buffer and which leaves the
+// CFL accumulator alone.
+template <typename R>
+static int upload_jobs(cns_batch *b, BatchGroup &g, int L, bool scratch) {
+    const size_t nm = g.m.size();
+    std::vector<Job<R>> h(nm * g.nst);
+    int first_total = 0;
+    for (int st = 0; st < g.nst; st++) {
+        int first = 0;
+        for (size_t k = 0; k < nm; k++) {
+            cns_solver *m = g.m[k];
+            const size_t mi = std::find(b->members.begin(), b->members.end(), m) - b->members.begin();
+            const int c0 = b->cur0[mi];
+            // A[p] serves steps whose count has parity p; the member's state is
+            // in set c0 at its current count
+            const int p0 = (int)(b->phase0[mi] & 1);
+            Job<R> &J = h[st * nm + k];
+            std::memset(&J, 0, sizeof(J));
+            Params P1;
+            stage_setup<R>(m, c0, st, true, J.P, J.A[p0]);
+            stage_setup<R>(m, next_cur(m, c0), st, true, P1, J.A[1 - p0]);
+            int zc, nzc, nw;
+            member_geometry(m, L, zc, nzc, nw);
+            J.P.zc = zc;
+            J.P.nzc = nzc;
+            J.P.xcd_remap = 0;
+            if (scratch) {
+                J.P.want_visc_max = 0;
+                for (int p = 0; p < 2; p++) {
+                    R *sb = as<R>(m->rbuf[0]);
+                    J.A[p].qout = J.A[p].rout = J.A[p].accout = J.A[p].qoutlo = J.A[p].accoutlo = sb;
+                }
+            }
+            J.dil = as<R>(m->d_dil);
+            J.sc = m->d_sc;
+            J.ctl = m->d_ctl;
+            J.part = m->d_part;
+            J.first = first;
+            J.nwork = nw;
+            first += nw;
+        }
+        first_total = first;
+    }
+    g.nwork = first_total;
+    g.job_size = sizeof(Job<R>);
+    if (!g.d_jobs) BCHK(b, hipMalloc(&g.d_jobs, h.size() * sizeof(Job<R>)));
+    BCHK(b, hipMemcpyAsync(g.d_jobs, h.data(), h.size() * sizeof(Job<R>), hipMemcpyHostToDevice, b->stream));
+    BCHK(b, hipStreamSynchronize(b->stream));
+    return CNS_OK;
+}
+
+static int upload(cns_batch *b, BatchGroup &g, int L, bool scratch) {
+    return g.fp32 ? upload_jobs<float>(b, g, L, scratch) : upload_jobs<double>(b, g, L, scratch);
+}
+
+static int blaunched(cns_batch *b, const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && b->sync_check) e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) {
+        b->err = std::string(what) + ": " + hipGetErrorString(e);
+        return CNS_ERR_HIP;
+    }
+    return CNS_OK;
+}
+#define BLAUNCHED(b, what)                  \
+    do {                                    \
+        int rc_ = blaunched((b), (what));   \
+        if (rc_) return rc_;                \
+    } while (0)
+
+// dilatation + RHS launches of stage st of group g
+template <typename R>
+static int queue_stage(cns_batch *b, const BatchGroup &g, int st) {
+    const int nm = (int)g.m.size();
+    const Job<R> *jobs = static_cast<const Job<R> *>(g.d_jobs) + (size_t)st * nm;
+    const int grid = std::max(1, std::min(g.grid, g.nwork));
+    BCHK(b, launch_dil_batch<R>(g.SV, g.wallx, g.nonuni, grid, b->stream, jobs, nm, g.nwork, g.remap));
+    BLAUNCHED(b, "k_dil_batch");
+    BCHK(b, launch_rhs_batch<R>(g.S, g.SV, g.wallx, g.nonuni, g.unitv, grid, b->stream, jobs, nm, g.nwork,
+                                g.remap));
+    BLAUNCHED(b, "k_rhs_batch");
+    return CNS_OK;
+}
+
+// one step of group g: control, then the stages
+template <typename R>
+static int queue_group_step(cns_batch *b, const BatchGroup &g) {
+    const int nm = (int)g.m.size();
+    const Job<R> *jobs = static_cast<const Job<R> *>(g.d_jobs);
+    const int lanes = (nm + 63) / 64;
+    hipLaunchKernelGGL(k_step_begin<R>, dim3(lanes), dim3(64), 0, b->stream, jobs, nm);
+    BLAUNCHED(b, "k_step_begin");
+    hipLaunchKernelGGL(k_cfl_batch<R>, dim3(256, nm), dim3(RED_THREADS), 0, b->stream, jobs);
+    BLAUNCHED(b, "k_cfl_batch");
+    if (g.forcing) {
+        hipLaunchKernelGGL(k_partials_batch<R>, dim3(RED_BLOCKS, nm), dim3(RED_THREADS), 0, b->stream, jobs);
+        BLAUNCHED(b, "k_partials_batch");
+    }
+    hipLaunchKernelGGL(k_step_mid<R>, dim3(nm), dim3(RED_THREADS), 0, b->stream, jobs);
+    BLAUNCHED(b, "k_step_mid");
+    for (int st = 0; st < g.nst; st++) {
+        int rc = queue_stage<R>(b, g, st);
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL(k_step_end<R>, dim3(lanes), dim3(64), 0, b->stream, jobs, nm);
+    BLAUNCHED(b, "k_step_end");
+    return CNS_OK;
+}
+
+static int queue_batch_step(cns_batch *b) {
+    for (const BatchGroup &g : b->groups) {
+        const int rc = g.fp32 ? queue_group_step<float>(b, g) : queue_group_step<double>(b, g);
+        if (rc) return rc;
+    }
+    return CNS_OK;
+}
+
+static void drop_graph(cns_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+// capture one batch step as a graph (launch errors surface at capture end)
+static int capture(cns_batch *b) {
+    drop_graph(b);
+    if (b->sync_check) return CNS_OK;  // checked mode launches eagerly
+    BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int rc = queue_batch_step(b);
+    hipGraph_t gr = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->stream, &gr);
+    if (rc) {
+        if (gr) (void)hipGraphDestroy(gr);
+        return rc;
+    }
+    if (e != hipSuccess) return bfail(b, CNS_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    b->graph = gr;
+    BCHK(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    return CNS_OK;
+}
+
+extern "C" int cns_batch_create(cns_solver *const *members, int n, cns_batch **out) {
+    if (!members || n <= 0 || !out) return bfail(nullptr, CNS_ERR_ARG, "cns_batch_create: bad arguments");
+    *out = nullptr;
+    for (int k = 0; k < n; k++) {
+        cns_solver *m = members[k];
+        if (!m) return bfail(nullptr, CNS_ERR_ARG, "cns_batch_create: null member");
+        if (!m->have_state) return bfail(nullptr, CNS_ERR_STATE, "cns_batch_create: member without a state");
+        if (m->in_batch) return bfail(nullptr, CNS_ERR_STATE, "cns_batch_create: member already in a batch");
+        if (m->dev != members[0]->dev) return bfail(nullptr, CNS_ERR_ARG, "cns_batch_create: members on different devices");
+        for (int q = 0; q < k; q++)
+            if (members[q] == m) return bfail(nullptr, CNS_ERR_ARG, "cns_batch_create: repeated member");
+    }
+    cns_batch *b = new cns_batch();
+    b->dev = members[0]->dev;
+    {
+        const char *sc = std::getenv("CNS_SYNC_CHECK");
+        b->sync_check = sc && sc[0] == '1';
+    }
+    auto bail = [&](int code) {
+        g_err = b->err;
+        cns_batch_destroy(b);
+        return code;
+    };
+    if (hipSetDevice(b->dev) != hipSuccess) {
+        b->err = "hipSetDevice failed";
+        return bail(CNS_ERR_HIP);
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, b->dev) == hipSuccess && prop.multiProcessorCount > 0)
+        b->cus = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        b->err = "hipStreamCreate failed";
+        return bail(CNS_ERR_HIP);
+    }
+    for (int k = 0; k < n; k++) {
+        cns_solver *m = members[k];
+        if (hipStreamSynchronize(m->stream) != hipSuccess) {
+            b->err = "member stream synchronisation failed";
+            return bail(CNS_ERR_HIP);
+        }
+        b->members.push_back(m);
+        b->cur0.push_back(m->cur);
+        b->phase0.push_back(m->bsteps);
+        // group key: kernel template and scheme
+        const bool fp32 = m->fp32, wallx = !m->cfg.periodic_x, nonuni = m->cfg.nonuniform_x != 0,
+                   unitv = m->cfg.visc_exp == 1.0;
+        BatchGroup *g = nullptr;
+        for (BatchGroup &x : b->groups)
+            if (x.fp32 == fp32 && x.S == m->S && x.SV == m->SV && x.wallx == wallx && x.nonuni == nonuni &&
+                x.unitv == unitv && x.scheme == m->cfg.scheme)
+                g = &x;
+        if (!g) {
+            b->groups.emplace_back();
+            g = &b->groups.back();
+            g->fp32 = fp32;
+            g->S = m->S;
+            g->SV = m->SV;
+            g->wallx = wallx;
+            g->nonuni = nonuni;
+            g->unitv = unitv;
+            g->scheme = m->cfg.scheme;
+            g->nst = n_stages(m);
+            g->grid = 4 * b->cus;
+        }
+        g->m.push_back(m);
+        g->forcing = g->forcing || m->cfg.forcing_z;
+        // step control: the count continues from the member's earlier batches
+        StepCtl c{};
+        c.istep = (int)m->bsteps;
+        c.vfs = m->visc_from_state ? 1 : 0;
+        c.check_cfl = m->cfg.check_cfl;
+        c.forcing = m->cfg.forcing_z;
+        c.cfl = m->cfg.cfl;
+        c.vol = m->cfg.len_x * m->cfg.len_y * m->cfg.len_z;
+        if (hipMemcpy(m->d_ctl, &c, sizeof(c), hipMemcpyHostToDevice) != hipSuccess) {
+            b->err = "step control upload failed";
+            return bail(CNS_ERR_HIP);
+        }
+    }
+    for (BatchGroup &g : b->groups) {
+        int rc = upload(b, g, g.pencil, false);
+        if (rc) return bail(rc);
+    }
+    int rc = capture(b);
+    if (rc) return bail(rc);
+    for (cns_solver *m : b->members) m->in_batch = true;
+    *out = b;
+    return CNS_OK;
+}
+
+// Time of stage `st` of group g (dilatation + RHS, scratch outputs), best of `reps`.
+template <typename R>
+static int time_stage(cns_batch *b, const BatchGroup &g, int st, int reps, float &best) {
+    hipEvent_t e0, e1;
+    BCHK(b, hipEventCreate(&e0));
+    BCHK(b, hipEventCreate(&e1));
+    best = 1e30f;
+    int rc = CNS_OK;
+    for (int r = 0; r < reps && !rc; r++) {
+        BCHK(b, hipEventRecord(e0, b->stream));
+        rc = queue_stage<R>(b, g, st);
+        if (rc) break;
+        BCHK(b, hipEventRecord(e1, b->stream));
+        BCHK(b, hipEventSynchronize(e1));
+        float ms = 0.f;
+        BCHK(b, hipEventElapsedTime(&ms, e0, e1));
+        best = std::min(best, ms);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
+static int time_stage_any(cns_batch *b, const BatchGroup &g, float &best) {
+    const int st = g.nst / 2;
+    return g.fp32 ? time_stage<float>(b, g, st, 3, best) : time_stage<double>(b, g, st, 3, best);
+}
+
+extern "C" int cns_batch_tune(cns_batch *b) {
+    if (!b) return bfail(nullptr, CNS_ERR_ARG, "null batch");
+    BCHK(b, hipSetDevice(b->dev));
+    for (BatchGroup &g : b->groups) {
+        int maxz = 0;
+        for (cns_solver *m : g.m) maxz = std::max(maxz, m->cfg.nz);
+        // 1. pencil length (z-chunk of a workgroup's march)
+        std::vector<int> pencils;
+        for (int L : {8, 16, 32, 64, 128, 256, 512})
+            if (L >= 2 * g.S && L <= std::max(maxz, 2 * g.S)) pencils.push_back(L);
+        if (pencils.empty()) pencils.push_back(maxz);
+        int bestL = g.pencil;
+        float bt = 1e30f;
+        g.grid = 4 * b->cus;
+        g.remap = 1;
+        for (int L : pencils) {
+            int rc = upload(b, g, L, true);
+            if (rc) return rc;
+            float t;
+            if ((rc = time_stage_any(b, g, t))) return rc;
+            if (t < bt) {
+                bt = t;
+                bestL = L;
+            }
+        }
+        g.pencil = bestL;
+        int rc = upload(b, g, g.pencil, true);
+        if (rc) return rc;
+        // 2. persistent workgroups (multiples of the CU count keep whole XCDs busy)
+        int bestG = g.grid;
+        bt = 1e30f;
+        for (int f : {1, 2, 4, 8}) {
+            g.grid = f * b->cus;
+            float t;
+            if ((rc = time_stage_any(b, g, t))) return rc;
+            if (t < bt) {
+                bt = t;
+                bestG = g.grid;
+            }
+        }
+        g.grid = bestG;
+        // 3. XCD-contiguous item order on or off
+        float t0, t1;
+        g.remap = 0;
+        if ((rc = time_stage_any(b, g, t0))) return rc;
+        g.remap = 1;
+        if ((rc = time_stage_any(b, g, t1))) return rc;
+        g.remap = t1 <= t0 ? 1 : 0;
+        if ((rc = upload(b, g, g.pencil, false))) return rc;
+    }
+    return capture(b);
+}
+
+static int batch_run(cns_batch *b, int nsteps) {
+    for (int i = 0; i < nsteps; i++) {
+        if (b->exec) {
+            BCHK(b, hipGraphLaunch(b->exec, b->stream));
+        } else {
+            int rc = queue_batch_step(b);
+            if (rc) return rc;
+        }
+    }
+    return CNS_OK;
+}
+
+static void batch_after(cns_batch *b, int nsteps) {
+    b->steps += nsteps;
+    if (nsteps > 0)
+        for (size_t k = 0; k < b->members.size(); k++) {
+            cns_solver *m = b->members[k];
+            m->visc_from_state = false;
+            m->cur = (b->steps & 1) ? next_cur(m, b->cur0[k]) : b->cur0[k];
+            m->bsteps = b->phase0[k] + b->steps;
+        }
+}
+
+extern "C" int cns_batch_step(cns_batch *b, int nsteps) {
+    if (!b) return bfail(nullptr, CNS_ERR_ARG, "null batch");
+    if (nsteps < 0) return bfail(b, CNS_ERR_ARG, "cns_batch_step: nsteps < 0");
+    BCHK(b, hipSetDevice(b->dev));
+    int rc = batch_run(b, nsteps);
+    if (rc) return rc;
+    BCHK(b, hipStreamSynchronize(b->stream));
+    batch_after(b, nsteps);
+    return CNS_OK;
+}
+
+extern "C" int cns_batch_time_steps(cns_batch *b, int nsteps, double *ms) {
+    if (!b || !ms) return bfail(b, CNS_ERR_ARG, "cns_batch_time_steps: null argument");
+    if (nsteps < 0) return bfail(b, CNS_ERR_ARG, "cns_batch_time_steps: nsteps < 0");
+    BCHK(b, hipSetDevice(b->dev));
+    hipEvent_t e0, e1;
+    BCHK(b, hipEventCreate(&e0));
+    BCHK(b, hipEventCreate(&e1));
+    BCHK(b, hipEventRecord(e0, b->stream));
+    int rc = batch_run(b, nsteps);
+    if (!rc) {
+        BCHK(b, hipEventRecord(e1, b->stream));
+        BCHK(b, hipEventSynchronize(e1));
+        float t = 0.f;
+        BCHK(b, hipEventElapsedTime(&t, e0, e1));
+        *ms = t;
+        batch_after(b, nsteps);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
+extern "C" int cns_batch_groups(const cns_batch *b) { return b ? (int)b->groups.size() : 0; }
+
+extern "C" int cns_batch_group_info(const cns_batch *b, int g, int *n_members, int *pencil, int *grid, int *remap,
+                                    int *work_items) {
+    if (!b || g < 0 || g >= (int)b->groups.size()) return CNS_ERR_ARG;
+    const BatchGroup &x = b->groups[g];
+    if (n_members) *n_members = (int)x.m.size();
+    if (pencil) *pencil = x.pencil;
+    if (grid) *grid = x.grid;
+    if (remap) *remap = x.remap;
+    if (work_items) *work_items = x.nwork;
+    return CNS_OK;
+}
+
+extern "C" int cns_batch_destroy(cns_batch *b) {
+    if (!b) return CNS_OK;
+    (void)hipSetDevice(b->dev);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    drop_graph(b);
+    for (BatchGroup &g : b->groups)
+        if (g.d_jobs) (void)hipFree(g.d_jobs);
+    for (cns_solver *m : b->members) m->in_batch = false;
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return CNS_OK;
+}
+
+extern "C" const char *cns_batch_last_error(const cns_batch *b) { return b ? b->err.c_str() : g_err.c_str(); }

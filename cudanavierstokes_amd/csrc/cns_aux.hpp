@@ -77,49 +77,56 @@ __device__ __forceinline__ void block_max2_to(double a, double b, unsigned long 
 //   solver loop otherwise takes mu from the last RHS input, as the reference
 //   reads the mu field of the last calcState)
 template <typename R>
+__device__ __forceinline__ void cfl_point(const Params &P, const StagePtrs<R> &A, long g, int visc_from_state,
+                                          double &cmax, double &vmax) {
+    const int i = (int)(g % P.mx);
+    const long fs = A.fs;
+    // full values (the offsets are zero for FP64 storage)
+    const double r = (double)A.qin[g] + P.rho0;
+    const double u = (double)A.qin[g + fs] / r, v = (double)A.qin[g + 2 * fs] / r, w = (double)A.qin[g + 3 * fs] / r;
+    const double ien = ((double)A.qin[g + 4 * fs] + P.E0) / r - 0.5 * (u * u + v * v + w * w);
+    const double sos = sqrt(P.gam * (P.gam - 1) * ien);
+    const double dx = P.dxv[i];
+    const double c1 = (fabs(u) + sos) / dx, c2 = (fabs(v) + sos) * P.d_dy, c3 = (fabs(w) + sos) * P.d_dz;
+    cmax = dmax(cmax, dmax(c1, dmax(c2, c3)));
+    if (visc_from_state) {
+        const double t = P.cvInv * ien;
+        const double mu = (P.unit_visc ? t : pow(t, P.viscexp)) / P.Re;
+        const double v1 = mu / (dx * dx), v2 = mu * P.d_d2y, v3 = mu * P.d_d2z;
+        vmax = dmax(vmax, dmax(v1, dmax(v2, v3)));
+    }
+}
+
+template <typename R>
 __global__ void k_cfl(Params P, StagePtrs<R> A, int visc_from_state, Scalars *sc) {
     const long mxy = (long)P.mx * P.my, n = mxy * P.mz;
     double cmax = 0.0, vmax = 0.0;
-    for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x) {
-        const int i = (int)(g % P.mx);
-        const long fs = A.fs;
-        // full values (the offsets are zero for FP64 storage)
-        const double r = (double)A.qin[g] + P.rho0;
-        const double u = (double)A.qin[g + fs] / r, v = (double)A.qin[g + 2 * fs] / r, w = (double)A.qin[g + 3 * fs] / r;
-        const double ien = ((double)A.qin[g + 4 * fs] + P.E0) / r - 0.5 * (u * u + v * v + w * w);
-        const double sos = sqrt(P.gam * (P.gam - 1) * ien);
-        const double dx = P.dxv[i];
-        const double c1 = (fabs(u) + sos) / dx, c2 = (fabs(v) + sos) * P.d_dy, c3 = (fabs(w) + sos) * P.d_dz;
-        cmax = dmax(cmax, dmax(c1, dmax(c2, c3)));
-        if (visc_from_state) {
-            const double t = P.cvInv * ien;
-            const double mu = (P.unit_visc ? t : pow(t, P.viscexp)) / P.Re;
-            const double v1 = mu / (dx * dx), v2 = mu * P.d_d2y, v3 = mu * P.d_d2z;
-            vmax = dmax(vmax, dmax(v1, dmax(v2, v3)));
-        }
-    }
+    for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x)
+        cfl_point(P, A, g, visc_from_state, cmax, vmax);
     block_max2_to(cmax, vmax, &sc->conv_max, visc_from_state ? &sc->visc_max : nullptr);
 }
 
 // dt = min(1e5, CFL / max(conv, visc)), which equals the reference's
 // min over points of CFL / max(conv_i, visc_i) (division is monotone).
-__global__ void k_dt_finalize(double cfl, Scalars *sc) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        const double cm = __longlong_as_double((long long)sc->conv_max);
-        const double vm = __longlong_as_double((long long)sc->visc_max);
-        const double m = cm >= vm ? cm : vm;
-        const double val = cfl / m;
-        sc->dt = (100000.0 <= val) ? 100000.0 : val;
-        sc->conv_max = 0ull;
-    }
+__device__ __forceinline__ void dt_finalize(double cfl, Scalars *sc) {
+    const double cm = __longlong_as_double((long long)sc->conv_max);
+    const double vm = __longlong_as_double((long long)sc->visc_max);
+    const double m = cm >= vm ? cm : vm;
+    const double val = cfl / m;
+    sc->dt = (100000.0 <= val) ? 100000.0 : val;
+    sc->conv_max = 0ull;
 }
 
-__global__ void k_reset_visc(Scalars *sc) {
+static __global__ void k_dt_finalize(double cfl, Scalars *sc) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) dt_finalize(cfl, sc);
+}
+
+static __global__ void k_reset_visc(Scalars *sc) {
     if (threadIdx.x == 0 && blockIdx.x == 0) sc->visc_max = 0ull;
 }
 
 // time[istep] = time + dt (src/calc_stress.cu:12-14, src/cuda_math.cu:17-19)
-__global__ void k_advance_time(Scalars *sc, double *time_out, int istep) {
+static __global__ void k_advance_time(Scalars *sc, double *time_out, int istep) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         sc->time = sc->time + sc->dt;
         if (time_out) time_out[istep] = sc->time;
@@ -129,7 +136,7 @@ __global__ void k_advance_time(Scalars *sc, double *time_out, int istep) {
 // Per-block partial volume integrals, weight dxv[i]/d_dy/d_dz
 // (src/cuda_math.cu:143-171): rho, rho*w, rho*E, u.u.
 template <typename R>
-__global__ void k_partials(Params P, StagePtrs<R> A, double *part) {
+__device__ __forceinline__ void partials_block(const Params &P, const StagePtrs<R> &A, double *part) {
     const long mxy = (long)P.mx * P.my, n = mxy * P.mz;
     double s[NPART] = {0, 0, 0, 0};
     for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x) {
@@ -156,11 +163,16 @@ __global__ void k_partials(Params P, StagePtrs<R> A, double *part) {
         for (int c = 0; c < NPART; c++) part[c * RED_BLOCKS + blockIdx.x] = sh[c][0];
 }
 
+template <typename R>
+__global__ void k_partials(Params P, StagePtrs<R> A, double *part) {
+    partials_block(P, A, part);
+}
+
 // Final fixed-order sum of the partials, then either the pressure-gradient
 // controller (src/calc_stress.cu:16-18,98-120) or the bulk statistics
 // (src/calc_stress.cu:162-201).  what: 1 controller, 2 bulk.
-__global__ void k_final(const double *part, Scalars *sc, int what, int forcing, double vol, double *par1,
-                        double *par2, int istep) {
+__device__ __forceinline__ void final_sums(const double *part, Scalars *sc, int what, int forcing, double vol,
+                                           double *par1, double *par2, int istep) {
     __shared__ double sh[NPART][RED_THREADS];
     for (int c = 0; c < NPART; c++) {
         double a = 0.0;
@@ -191,7 +203,12 @@ __global__ void k_final(const double *part, Scalars *sc, int what, int forcing, 
     }
 }
 
-__global__ void k_fill(double *a, long n, double v) {
+static __global__ void k_final(const double *part, Scalars *sc, int what, int forcing, double vol, double *par1,
+                        double *par2, int istep) {
+    final_sums(part, sc, what, forcing, vol, par1, par2, istep);
+}
+
+static __global__ void k_fill(double *a, long n, double v) {
     for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x) a[g] = v;
 }
 

@@ -1,6 +1,8 @@
-// cns_kernels.hip -- instantiates the RHS and dilatation kernels for one
-// advective half width S = CNS_S (compiled once per S), every viscous half
-// width SV <= S and both x-boundary / x-grid variants.
+// cns_kernels.hip -- instantiates the RHS and dilatation kernels (single
+// instance and batched) for one advective half width S = CNS_S, every
+// viscous half width SV <= S and both x-boundary / x-grid variants; for one
+// storage type CNS_R (double or float), or both when CNS_R is not defined.
+#include "cns_batch.hpp"
 #include "cns_launch.hpp"
 #include "cns_rhs.hpp"
 
@@ -51,10 +53,63 @@ static hipError_t go_dil(bool wallx, bool nonuni, int nb, hipStream_t st, const 
     return hipGetLastError();
 }
 
+template <typename R, int S, int SV, bool UNITV>
+static hipError_t go_rhs_batch_v(bool wallx, bool nonuni, int grid, hipStream_t st, const Job<R> *jobs, int njobs,
+                                 int nwork, int remap) {
+    const dim3 g(grid), b(TILE_X * TILE_Y);
+    if (wallx) {
+        if (nonuni)
+            hipLaunchKernelGGL((k_rhs_batch<R, S, SV, TILE_X, TILE_Y, true, true, UNITV>), g, b, 0, st, jobs, njobs,
+                               nwork, remap);
+        else
+            hipLaunchKernelGGL((k_rhs_batch<R, S, SV, TILE_X, TILE_Y, true, false, UNITV>), g, b, 0, st, jobs, njobs,
+                               nwork, remap);
+    } else {
+        if (nonuni)
+            hipLaunchKernelGGL((k_rhs_batch<R, S, SV, TILE_X, TILE_Y, false, true, UNITV>), g, b, 0, st, jobs, njobs,
+                               nwork, remap);
+        else
+            hipLaunchKernelGGL((k_rhs_batch<R, S, SV, TILE_X, TILE_Y, false, false, UNITV>), g, b, 0, st, jobs,
+                               njobs, nwork, remap);
+    }
+    return hipGetLastError();
+}
+
+template <typename R, int S, int SV>
+static hipError_t go_rhs_batch(bool wallx, bool nonuni, bool unitv, int grid, hipStream_t st, const Job<R> *jobs,
+                               int njobs, int nwork, int remap) {
+    return unitv ? go_rhs_batch_v<R, S, SV, true>(wallx, nonuni, grid, st, jobs, njobs, nwork, remap)
+                 : go_rhs_batch_v<R, S, SV, false>(wallx, nonuni, grid, st, jobs, njobs, nwork, remap);
+}
+
+template <typename R, int SV>
+static hipError_t go_dil_batch(bool wallx, bool nonuni, int grid, hipStream_t st, const Job<R> *jobs, int njobs,
+                               int nwork, int remap) {
+    const dim3 g(grid), b(TILE_X * TILE_Y);
+    if (wallx) {
+        if (nonuni)
+            hipLaunchKernelGGL((k_dil_batch<R, SV, TILE_X, TILE_Y, true, true>), g, b, 0, st, jobs, njobs, nwork,
+                               remap);
+        else
+            hipLaunchKernelGGL((k_dil_batch<R, SV, TILE_X, TILE_Y, true, false>), g, b, 0, st, jobs, njobs, nwork,
+                               remap);
+    } else {
+        if (nonuni)
+            hipLaunchKernelGGL((k_dil_batch<R, SV, TILE_X, TILE_Y, false, true>), g, b, 0, st, jobs, njobs, nwork,
+                               remap);
+        else
+            hipLaunchKernelGGL((k_dil_batch<R, SV, TILE_X, TILE_Y, false, false>), g, b, 0, st, jobs, njobs, nwork,
+                               remap);
+    }
+    return hipGetLastError();
+}
+
 #define CNS_CAT2(a, b) a##b
 #define CNS_CAT(a, b) CNS_CAT2(a, b)
 #define LAUNCH_RHS CNS_CAT(launch_rhs_s, CNS_S)
 #define LAUNCH_DIL CNS_CAT(launch_dil_s, CNS_S)
+#define LAUNCH_RHS_BATCH CNS_CAT(launch_rhs_batch_s, CNS_S)
+#define LAUNCH_DIL_BATCH CNS_CAT(launch_dil_batch_s, CNS_S)
 
 template <typename R>
 hipError_t LAUNCH_RHS(int sv, bool wallx, bool nonuni, int nb, hipStream_t st, const Params &P,
@@ -83,11 +138,45 @@ hipError_t LAUNCH_DIL(bool wallx, bool nonuni, int nb, hipStream_t st, const Par
     return go_dil<R, CNS_S>(wallx, nonuni, nb, st, P, A, dil);
 }
 
-template hipError_t LAUNCH_RHS<double>(int, bool, bool, int, hipStream_t, const Params &, const StagePtrs<double> &);
-template hipError_t LAUNCH_DIL<double>(bool, bool, int, hipStream_t, const Params &, const StagePtrs<double> &,
-                                       double *);
-template hipError_t LAUNCH_RHS<float>(int, bool, bool, int, hipStream_t, const Params &, const StagePtrs<float> &);
-template hipError_t LAUNCH_DIL<float>(bool, bool, int, hipStream_t, const Params &, const StagePtrs<float> &,
-                                      float *);
+template <typename R>
+hipError_t LAUNCH_RHS_BATCH(int sv, bool wallx, bool nonuni, bool unitv, int grid, hipStream_t st, const Job<R> *jobs,
+                            int njobs, int nwork, int remap) {
+    switch (sv) {
+#if CNS_S >= 1
+    case 1: return go_rhs_batch<R, CNS_S, 1>(wallx, nonuni, unitv, grid, st, jobs, njobs, nwork, remap);
+#endif
+#if CNS_S >= 2
+    case 2: return go_rhs_batch<R, CNS_S, 2>(wallx, nonuni, unitv, grid, st, jobs, njobs, nwork, remap);
+#endif
+#if CNS_S >= 3
+    case 3: return go_rhs_batch<R, CNS_S, 3>(wallx, nonuni, unitv, grid, st, jobs, njobs, nwork, remap);
+#endif
+#if CNS_S >= 4
+    case 4: return go_rhs_batch<R, CNS_S, 4>(wallx, nonuni, unitv, grid, st, jobs, njobs, nwork, remap);
+#endif
+    }
+    return hipErrorInvalidValue;
+}
+
+// here CNS_S is the viscous half width of the dilatation kernel
+template <typename R>
+hipError_t LAUNCH_DIL_BATCH(bool wallx, bool nonuni, int grid, hipStream_t st, const Job<R> *jobs, int njobs,
+                            int nwork, int remap) {
+    return go_dil_batch<R, CNS_S>(wallx, nonuni, grid, st, jobs, njobs, nwork, remap);
+}
+
+#define CNS_INSTANTIATE(R)                                                                                        \
+    template hipError_t LAUNCH_RHS<R>(int, bool, bool, int, hipStream_t, const Params &, const StagePtrs<R> &);  \
+    template hipError_t LAUNCH_DIL<R>(bool, bool, int, hipStream_t, const Params &, const StagePtrs<R> &, R *);  \
+    template hipError_t LAUNCH_RHS_BATCH<R>(int, bool, bool, bool, int, hipStream_t, const Job<R> *, int, int,    \
+                                            int);                                                                 \
+    template hipError_t LAUNCH_DIL_BATCH<R>(bool, bool, int, hipStream_t, const Job<R> *, int, int, int);
+
+#if defined(CNS_R)
+CNS_INSTANTIATE(CNS_R)
+#else
+CNS_INSTANTIATE(double)
+CNS_INSTANTIATE(float)
+#endif
 
 }  // namespace cns

@@ -2,7 +2,7 @@
 // advective stencil half width S is compiled in its own translation unit
 // (cns_kernels.hip built with -DCNS_S=1..4).
 #pragma once
-#include "cns_device.hpp"
+#include "cns_batch.hpp"
 
 namespace cns {
 
@@ -15,7 +15,13 @@ constexpr int TILE_Y = 16;
                                const StagePtrs<R> &A);                                                         \
     template <typename R>                                                                                      \
     hipError_t launch_dil_s##N(bool wallx, bool nonuni, int nblocks, hipStream_t st, const Params &P,         \
-                               const StagePtrs<R> &A, R *dil);
+                               const StagePtrs<R> &A, R *dil);                                                 \
+    template <typename R>                                                                                      \
+    hipError_t launch_rhs_batch_s##N(int sv, bool wallx, bool nonuni, bool unitv, int grid, hipStream_t st,    \
+                                     const Job<R> *jobs, int njobs, int nwork, int remap);                    \
+    template <typename R>                                                                                      \
+    hipError_t launch_dil_batch_s##N(bool wallx, bool nonuni, int grid, hipStream_t st, const Job<R> *jobs,   \
+                                     int njobs, int nwork, int remap);
 CNS_DECLARE_LAUNCH(1)
 CNS_DECLARE_LAUNCH(2)
 CNS_DECLARE_LAUNCH(3)
@@ -43,6 +49,30 @@ inline hipError_t launch_dil(int sv, bool wallx, bool nonuni, int nblocks, hipSt
     case 2: return launch_dil_s2<R>(wallx, nonuni, nblocks, st, P, A, dil);
     case 3: return launch_dil_s3<R>(wallx, nonuni, nblocks, st, P, A, dil);
     case 4: return launch_dil_s4<R>(wallx, nonuni, nblocks, st, P, A, dil);
+    }
+    return hipErrorInvalidValue;
+}
+
+template <typename R>
+inline hipError_t launch_rhs_batch(int s, int sv, bool wallx, bool nonuni, bool unitv, int grid, hipStream_t st,
+                                   const Job<R> *jobs, int njobs, int nwork, int remap) {
+    switch (s) {
+    case 1: return launch_rhs_batch_s1<R>(sv, wallx, nonuni, unitv, grid, st, jobs, njobs, nwork, remap);
+    case 2: return launch_rhs_batch_s2<R>(sv, wallx, nonuni, unitv, grid, st, jobs, njobs, nwork, remap);
+    case 3: return launch_rhs_batch_s3<R>(sv, wallx, nonuni, unitv, grid, st, jobs, njobs, nwork, remap);
+    case 4: return launch_rhs_batch_s4<R>(sv, wallx, nonuni, unitv, grid, st, jobs, njobs, nwork, remap);
+    }
+    return hipErrorInvalidValue;
+}
+
+template <typename R>
+inline hipError_t launch_dil_batch(int sv, bool wallx, bool nonuni, int grid, hipStream_t st, const Job<R> *jobs,
+                                   int njobs, int nwork, int remap) {
+    switch (sv) {
+    case 1: return launch_dil_batch_s1<R>(wallx, nonuni, grid, st, jobs, njobs, nwork, remap);
+    case 2: return launch_dil_batch_s2<R>(wallx, nonuni, grid, st, jobs, njobs, nwork, remap);
+    case 3: return launch_dil_batch_s3<R>(wallx, nonuni, grid, st, jobs, njobs, nwork, remap);
+    case 4: return launch_dil_batch_s4<R>(wallx, nonuni, grid, st, jobs, njobs, nwork, remap);
     }
     return hipErrorInvalidValue;
 }

@@ -314,7 +314,7 @@ __device__ __forceinline__ long long uniform64(long long v) {
 // reference state and the update is evaluated in FP64 on the compensated
 // pair hi + lo, rounded back to hi and lo (F_COMP).
 template <typename R, int S, int SV, int TX, int TY, bool WALLX, bool NONUNI, bool UNITV>
-__global__ __launch_bounds__(TX *TY) void k_rhs(Params P, StagePtrs<R> A) {
+__device__ __forceinline__ void rhs_tile(const Params &P, const StagePtrs<R> &A, const int bb) {
     constexpr int H = S, LX = TX + 2 * H, LY = TY + 2 * H, NT = TX * TY, W = 2 * H + 1;
     constexpr int NL = UNITV ? NQ - 1 : NQ;  // staged quantities (mu rebuilt from T when mu = T/Re)
     constexpr int NH = 2 * H * TY + 2 * H * TX;
@@ -356,7 +356,6 @@ __global__ __launch_bounds__(TX *TY) void k_rhs(Params P, StagePtrs<R> A) {
         return reinterpret_cast<R *>(const_cast<char *>(reinterpret_cast<const char *>(A.qin)) + o) + poff;
     };
 
-    const int bb = remap_block(blockIdx.x, gridDim.x, P.xcd_remap);
     const int tile_x = bb % P.ntx;
     const int rest = bb / P.ntx;
     const int tile_y = rest % P.nty;
@@ -633,18 +632,22 @@ __global__ __launch_bounds__(TX *TY) void k_rhs(Params P, StagePtrs<R> A) {
     if (P.want_visc_max) block_max_atomic(vmax, P.visc_max_out, red);
 }
 
+template <typename R, int S, int SV, int TX, int TY, bool WALLX, bool NONUNI, bool UNITV>
+__global__ __launch_bounds__(TX *TY) void k_rhs(Params P, StagePtrs<R> A) {
+    rhs_tile<R, S, SV, TX, TY, WALLX, NONUNI, UNITV>(P, A, remap_block(blockIdx.x, gridDim.x, P.xcd_remap));
+}
+
 // Dilatation of the stage input: du/dx + dv/dy + dw/dz at viscous order
 // (src/calc_stress.cu:20-63, 87-96).  Same tiling and prefetch scheme as
 // k_rhs with halo SV: u and v of plane k (tile + halo) in double-buffered
 // LDS, w of the own column in a register window.
 template <typename R, int SV, int TX, int TY, bool WALLX, bool NONUNI>
-__global__ __launch_bounds__(TX *TY) void k_dil(Params P, StagePtrs<R> A, R *dil) {
+__device__ __forceinline__ void dil_tile(const Params &P, const StagePtrs<R> &A, R *dil, const int bb) {
     constexpr int H = SV, LX = TX + 2 * H, LY = TY + 2 * H, NT = TX * TY, W = 2 * H + 1;
     constexpr int NH = 2 * H * TY + 2 * H * TX;
     static_assert(NH <= NT, "one halo point per lane");
     constexpr unsigned ESZ = sizeof(R);
     __shared__ R su[2][LY * LX], sv[2][LY * LX];
-    const int bb = remap_block(blockIdx.x, gridDim.x, P.xcd_remap);
     const int tile_x = bb % P.ntx;
     const int rest = bb / P.ntx;
     const int tile_y = rest % P.nty;
@@ -752,6 +755,11 @@ __global__ __launch_bounds__(TX *TY) void k_dil(Params P, StagePtrs<R> A, R *dil
 #pragma unroll
         for (int a = 0; a < W - 1; a++) ww[a] = ww[a + 1];
     }
+}
+
+template <typename R, int SV, int TX, int TY, bool WALLX, bool NONUNI>
+__global__ __launch_bounds__(TX *TY) void k_dil(Params P, StagePtrs<R> A, R *dil) {
+    dil_tile<R, SV, TX, TY, WALLX, NONUNI>(P, A, dil, remap_block(blockIdx.x, gridDim.x, P.xcd_remap));
 }
 
 }  // namespace cns

@@ -18,7 +18,7 @@ from dataclasses import dataclass, fields
 import numpy as np
 
 from . import _lib
-from ._lib import CnsConfig, CnsError, check
+from ._lib import CnsConfig, CnsError, check, check_batch
 
 SCHEMES = {"rk3": 0, "smr_rk3": 0, "lowstorage": 0, "kutta_rk3": 1, "rk4": 2}
 
@@ -198,6 +198,63 @@ class Solver:
         b = C.c_uint64()
         self._chk(self.L.cns_device_bytes(self.h, C.byref(b)), "cns_device_bytes")
         return b.value
+
+
+class Batch:
+    """Many solvers advanced together (cns_batch_*): one launch per RK stage
+    per kernel-template group, one captured graph per step.  A member's
+    result equals its own cns_step result bitwise; members cannot be stepped
+    alone while the batch exists."""
+
+    def __init__(self, solvers, tune=False):
+        self.L = _lib.load()
+        self.members = list(solvers)
+        arr = (C.c_void_p * len(self.members))(*[s.h for s in self.members])
+        h = C.c_void_p()
+        check_batch(self.L.cns_batch_create(arr, len(self.members), C.byref(h)), None, "cns_batch_create")
+        self.h = h
+        if tune:
+            self.tune()
+
+    def _chk(self, rc, what):
+        check_batch(rc, self.h, what)
+
+    def tune(self):
+        self._chk(self.L.cns_batch_tune(self.h), "cns_batch_tune")
+        return self.groups()
+
+    def step(self, nsteps):
+        self._chk(self.L.cns_batch_step(self.h, nsteps), "cns_batch_step")
+
+    def time_steps(self, nsteps):
+        ms = C.c_double()
+        self._chk(self.L.cns_batch_time_steps(self.h, nsteps, C.byref(ms)), "cns_batch_time_steps")
+        return ms.value
+
+    def groups(self):
+        out = []
+        for g in range(self.L.cns_batch_groups(self.h)):
+            v = [C.c_int() for _ in range(5)]
+            self._chk(self.L.cns_batch_group_info(self.h, g, *[C.byref(x) for x in v]), "cns_batch_group_info")
+            out.append(dict(zip(("members", "pencil", "grid", "remap", "work_items"), [x.value for x in v])))
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.cns_batch_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------------------

@@ -136,6 +136,38 @@ int cns_time_steps(cns_solver *s, int n_steps, double *ms);
 /* bytes of device memory held by the solver */
 int cns_device_bytes(const cns_solver *s, uint64_t *bytes);
 
+/* ---- Batched advance of many solvers (ensembles, the serving loop) ----
+ * No counterpart in the reference, which runs one flow per process.  A batch
+ * advances its members together: members that share a kernel template
+ * (stencil widths, x boundary and grid kind, viscosity law, storage
+ * precision) and time scheme form a group, and each group advances with one
+ * launch per RK stage covering every member's tiles (plus one dilatation
+ * launch per stage and four small control launches per step).  A step of the
+ * batch is captured once as a hipGraph and replayed.  Every member follows
+ * exactly the arithmetic of cns_step on its own, so a member's result after
+ * n batched steps equals its result after cns_step(n) bitwise.  Per member,
+ * steps are counted from its last cns_set_state or cns_step, across
+ * successive batches (the CFL check and forcing controller run on steps 0,
+ * check_cfl, 2 check_cfl, ...), so a member may move between batches.
+ * Members must have a state, live on one device and not be used on their
+ * own while the batch exists. */
+typedef struct cns_batch cns_batch;
+int cns_batch_create(cns_solver *const *members, int n_members, cns_batch **out);
+/* Pick per group the z-pencil length, the persistent workgroup count and the
+ * XCD-contiguous tile order by timing RHS launches whose outputs go to
+ * scratch buffers (the members' states are not changed); then re-captures. */
+int cns_batch_tune(cns_batch *b);
+/* n_steps steps of every member (graph replays on the batch stream) */
+int cns_batch_step(cns_batch *b, int n_steps);
+/* the same, timed with HIP events on the batch stream */
+int cns_batch_time_steps(cns_batch *b, int n_steps, double *ms);
+/* tuned launch shape of group g: pencil length, workgroups, XCD ordering, work items */
+int cns_batch_group_info(const cns_batch *b, int g, int *n_members, int *pencil, int *grid, int *remap,
+                         int *work_items);
+int cns_batch_groups(const cns_batch *b);
+int cns_batch_destroy(cns_batch *b);
+const char *cns_batch_last_error(const cns_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

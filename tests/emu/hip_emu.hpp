@@ -168,7 +168,7 @@ inline EmuPool emu_pool;
 inline void __syncthreads() { emu_bar->arrive_and_wait(); }
 
 template <class F, class... Args>
-void emu_launch(F f, dim3 grid, dim3 block, Args... args) {
+void emu_launch_now(F f, dim3 grid, dim3 block, Args... args) {
     const unsigned nt = block.x * block.y * block.z;
     emu_pool.ensure(nt < 64 ? 64 : nt);
     for (unsigned bz = 0; bz < grid.z; bz++)
@@ -246,7 +246,7 @@ inline void __syncthreads() {
 }
 
 template <class F, class... Args>
-void emu_launch(F f, dim3 grid, dim3 block, Args... args) {
+void emu_launch_now(F f, dim3 grid, dim3 block, Args... args) {
     const unsigned nt = block.x * block.y * block.z;
     emu_grid = grid;
     emu_block = block;
@@ -279,5 +279,53 @@ void emu_launch(F f, dim3 grid, dim3 block, Args... args) {
             }
 }
 #endif
+
+// Stream capture: launches made while capturing are recorded and replayed
+// by hipGraphLaunch (no other operation is captured by the product code).
+struct EmuGraph {
+    std::vector<std::function<void()>> ops;
+};
+typedef EmuGraph *hipGraph_t;
+typedef EmuGraph *hipGraphExec_t;
+enum { hipStreamCaptureModeThreadLocal = 2 };
+inline thread_local EmuGraph *emu_capturing = nullptr;
+inline hipError_t hipStreamBeginCapture(hipStream_t, int) {
+    emu_capturing = new EmuGraph();
+    return hipSuccess;
+}
+inline hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t *g) {
+    *g = emu_capturing;
+    emu_capturing = nullptr;
+    return hipSuccess;
+}
+inline hipError_t hipGraphInstantiate(hipGraphExec_t *e, hipGraph_t g, void *, void *, size_t) {
+    *e = g;
+    return hipSuccess;
+}
+inline hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t) {
+    for (auto &op : e->ops) op();
+    return hipSuccess;
+}
+inline hipError_t hipGraphExecDestroy(hipGraphExec_t) { return hipSuccess; }
+inline hipError_t hipGraphDestroy(hipGraph_t g) {
+    delete g;
+    return hipSuccess;
+}
+struct hipDeviceProp_t {
+    int multiProcessorCount;
+};
+// few "CUs" so that persistent launches give every workgroup several items
+inline hipError_t hipGetDeviceProperties(hipDeviceProp_t *p, int) {
+    p->multiProcessorCount = 8;
+    return hipSuccess;
+}
+
+template <class F, class... Args>
+void emu_launch(F f, dim3 grid, dim3 block, Args... args) {
+    if (emu_capturing)
+        emu_capturing->ops.push_back([=]() { emu_launch_now(f, grid, block, args...); });
+    else
+        emu_launch_now(f, grid, block, args...);
+}
 
 #define hipLaunchKernelGGL(F, G, B, SH, ST, ...) emu_launch(F, dim3(G), dim3(B), __VA_ARGS__)
