@@ -34,6 +34,9 @@ def parse_args():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-storage run")
     ap.add_argument("--err-steps", type=int, default=20, help="steps of the FP32-vs-FP64 error comparison")
+    ap.add_argument("--ensemble-steps", type=int, default=100, help="steps of the 64-instance ensemble (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=100,
+                    help="requests of the seeded serving stream to run (0: skip; the full stream has 1000)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0: host cores, at most 16)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (reference point for the "
@@ -94,6 +97,56 @@ def copy_calibration(mib, reps=5):
         dst.copy_(src)
     torch.cuda.synchronize()
     print(json.dumps({"copy_calibration_bytes": n * 8, "dispatches": reps}), flush=True)
+
+
+def ensemble_measure(cns, steps):
+    """64 seeded Taylor-Green instances (32^3..128^3, Re 400-3200, Mach
+    0.1-0.5), 6th order, RK4, FP32 storage: each alone in turn, then all in
+    one tuned batch (one launch per RK stage, graph replay)."""
+    from cudanavierstokes_amd.workloads import ensemble
+    inst = ensemble(n_instances=64, seed=4, steps=steps)
+    work = sum(i["config"].mx * i["config"].my * i["config"].mz for i in inst) * steps
+    seq_ms = 0.0
+    for i in inst:
+        s = cns.Solver(i["config"])
+        s.set_state(*s.taylor_green_fields())
+        s.step(1, outputs=False)
+        seq_ms += s.time_steps(steps)
+        s.close()
+    sols = []
+    for i in inst:
+        s = cns.Solver(i["config"])
+        s.set_state(*s.taylor_green_fields())
+        sols.append(s)
+    b = cns.Batch(sols)
+    t0 = time.perf_counter()
+    groups = b.tune()
+    tune_s = time.perf_counter() - t0
+    b.step(1)
+    bat_ms = b.time_steps(steps)
+    b.close()
+    for s in sols:
+        s.close()
+    seq = work / (seq_ms / 1e3) / 1e9
+    bat = work / (bat_ms / 1e3) / 1e9
+    return {"workload": "64 Taylor-Green instances, sizes {32,48,64,96,128}^3 (seed 4), order 6, RK4, "
+                        f"FP32 storage, {steps} steps each",
+            "sequential_gpt_steps_per_s": round(seq, 4), "batched_gpt_steps_per_s": round(bat, 4),
+            "speedup": round(bat / seq, 3), "sequential_ms": round(seq_ms, 2), "batched_ms": round(bat_ms, 2),
+            "tune_s": round(tune_s, 3), "groups": groups}
+
+
+def serve_measure(n_requests):
+    """The first n_requests of the seeded serving stream (Poisson, 20/s; 70%
+    Taylor-Green 48^3-160^3, 30% channel; 6th order, RK3, FP32) on the wall
+    clock, continuous batching within a 200 GB budget."""
+    from cudanavierstokes_amd.serve import Server
+    from cudanavierstokes_amd.workloads import serving
+    reqs = serving(n_requests=n_requests, seed=5, rate=20.0)
+    _, summary = Server(budget_bytes=200e9, tick_steps=10).run(reqs, realtime=True)
+    summary["workload"] = (f"first {n_requests} of 1000 seeded requests (Poisson 20/s), 70% Taylor-Green "
+                           "48^3-160^3 / 30% channel, order 6, RK3, FP32 storage, 200 GB admission budget")
+    return summary
 
 
 def main():
@@ -197,6 +250,10 @@ def main():
                          "kernel_ms_per_launch": round(m32["per_launch_ms"], 4)},
             "rel_err_vs_fp64": {"steps": a.err_steps, "l2_linf": fp32_error(a.err_steps)},
         }
+    if a.ensemble_steps > 0:
+        out["ensemble"] = ensemble_measure(cns, a.ensemble_steps)
+    if a.serve_requests > 0:
+        out["serving"] = serve_measure(a.serve_requests)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(n, order, a.cpu_threads or host_cores())
     print(json.dumps(out), flush=True)

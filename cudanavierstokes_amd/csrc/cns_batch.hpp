@@ -58,7 +58,7 @@ __device__ __forceinline__ int job_of(const Job<R> *__restrict__ jobs, int njobs
 // range of items (neighbouring tiles of one instance share halo planes in
 // that XCD's L2).
 template <typename R, int S, int SV, int TX, int TY, bool WALLX, bool NONUNI, bool UNITV>
-__global__ __launch_bounds__(TX *TY) void k_rhs_batch(const Job<R> *__restrict__ jobs, int njobs, int nwork,
+__global__ __launch_bounds__(TX *TY, (rhs_waves<R, UNITV>())) void k_rhs_batch(const Job<R> *__restrict__ jobs, int njobs, int nwork,
                                                       int remap) {
     for (int w0 = blockIdx.x; w0 < nwork; w0 += gridDim.x) {
         const int w = remap_block(w0, nwork, remap);

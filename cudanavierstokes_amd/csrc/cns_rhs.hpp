@@ -632,8 +632,17 @@ __device__ __forceinline__ void rhs_tile(const Params &P, const StagePtrs<R> &A,
     if (P.want_visc_max) block_max_atomic(vmax, P.visc_max_out, red);
 }
 
+// Waves per SIMD the RHS kernel is compiled for: FP32 storage with mu = T/Re
+// fits two (at most 256 VGPRs, with a few scratch spills, which cost less
+// than the latency the second wave hides); the other variants need more
+// registers than two waves can have and keep one.
+template <typename R, bool UNITV>
+constexpr int rhs_waves() {
+    return (sizeof(R) == 4 && UNITV) ? 2 : 1;
+}
+
 template <typename R, int S, int SV, int TX, int TY, bool WALLX, bool NONUNI, bool UNITV>
-__global__ __launch_bounds__(TX *TY) void k_rhs(Params P, StagePtrs<R> A) {
+__global__ __launch_bounds__(TX *TY, (rhs_waves<R, UNITV>())) void k_rhs(Params P, StagePtrs<R> A) {
     rhs_tile<R, S, SV, TX, TY, WALLX, NONUNI, UNITV>(P, A, remap_block(blockIdx.x, gridDim.x, P.xcd_remap));
 }
 

@@ -1,0 +1,162 @@
+"""Continuous-batching request loop for many simulations on one GPU.
+
+Requests (a configuration, an initial-condition kind, a step count and an
+arrival time) are admitted first-come first-served while the predicted device
+footprint of the admitted states stays within a budget; admitted states stay
+resident until their last step.  Every tick advances all admitted requests
+together with one batch (cns_batch_*: one launch per RK stage per kernel
+group, one replayed graph per step), by the smallest remaining step count or
+the tick length, whichever is less; finished requests leave, waiting ones are
+admitted, and the batch is rebuilt for the new membership.  Each request's
+result equals its lone cns_step result to rounding (the batch keeps every
+member's step count, so the CFL checks fall on the same steps).
+
+Run as a module for the seeded serving workload:
+    python -m cudanavierstokes_amd.serve --requests 1000 --rate 20
+"""
+import argparse
+import collections
+import json
+import time
+
+import numpy as np
+
+from .solver import Batch, Solver
+from .workloads import fields_for, points, serving
+
+_SETS = {0: (2, 2, 2), 1: (3, 1, 2), 2: (4, 2, 1)}  # scheme -> (state sets, compensated sets, rhs sets)
+
+
+def footprint(config):
+    """Device bytes a solver of this configuration allocates (the large
+    arrays; metrics and reduction scratch add well under a megabyte)."""
+    n = points(config)
+    sets, lo, rhs = _SETS[int(config.scheme)]
+    fp32 = int(config.precision) == 1
+    esz = 4 if fp32 else 8
+    fields = 5 * sets + (5 * lo if fp32 and config.compensated else 0) + 5 * rhs + 1
+    small = 8 * (config.mx * (2 * config.stencil_visc + 4) + config.mz + 5 * config.mx * config.mz + 4 * 1024) + 4096
+    return fields * n * esz + small
+
+
+class Server:
+    def __init__(self, budget_bytes=200e9, tick_steps=10, max_members=512, tune=False, keep_results=False):
+        self.budget = float(budget_bytes)
+        self.tick = int(tick_steps)
+        self.max_members = int(max_members)
+        self.tune = tune
+        self.keep = keep_results
+
+    def run(self, requests, realtime=True, log=None):
+        """Serve `requests` (dicts with id, arrival [s], config, kind, steps,
+        seed).  realtime: arrivals follow the wall clock; otherwise, when the
+        GPU is idle, the clock jumps to the next arrival.  Returns one record
+        per request and the run's summary."""
+        pending = collections.deque(sorted(requests, key=lambda r: r["arrival"]))
+        waiting = collections.deque()
+        active = {}
+        done = []
+        used = 0.0
+        batch = None
+        changed = False
+        shift = 0.0
+        t0 = time.perf_counter()
+        clock = lambda: time.perf_counter() - t0 + shift  # noqa: E731
+        busy = 0.0
+        ticks = 0
+        while pending or waiting or active:
+            now = clock()
+            while pending and pending[0]["arrival"] <= now:
+                waiting.append(pending.popleft())
+            while waiting and len(active) < self.max_members:
+                r = waiting[0]
+                need = footprint(r["config"])
+                if need > self.budget:
+                    raise ValueError(f"request {r['id']} needs {need:.3g} B, above the budget")
+                if used + need > self.budget:
+                    break
+                waiting.popleft()
+                s = Solver(r["config"])
+                s.set_state(*fields_for(s, r["kind"], r.get("seed", 1)))
+                active[r["id"]] = dict(req=r, solver=s, left=int(r["steps"]), admitted=clock(), bytes=need)
+                used += need
+                changed = True
+            if not active:
+                if pending:
+                    wait = pending[0]["arrival"] - clock()
+                    if wait > 0:
+                        if realtime:
+                            time.sleep(wait)
+                        else:
+                            shift += wait
+                continue
+            if changed or batch is None:
+                if batch is not None:
+                    batch.close()
+                batch = Batch([a["solver"] for a in active.values()], tune=self.tune)
+                changed = False
+            k = min(self.tick, min(a["left"] for a in active.values()))
+            tb = time.perf_counter()
+            batch.step(k)
+            busy += time.perf_counter() - tb
+            ticks += 1
+            finished = []
+            for rid, a in active.items():
+                a["left"] -= k
+                if a["left"] == 0:
+                    finished.append(rid)
+            if finished:
+                batch.close()
+                batch = None
+                for rid in finished:
+                    a = active.pop(rid)
+                    state = a["solver"].get_state()
+                    t_done = clock()
+                    r = a["req"]
+                    rec = dict(id=r["id"], kind=r["kind"], arrival=r["arrival"], admitted=a["admitted"],
+                               done=t_done, tts=t_done - r["arrival"], steps=int(r["steps"]),
+                               points=points(r["config"]))
+                    if self.keep:
+                        rec["state"] = state
+                    done.append(rec)
+                    a["solver"].close()
+                    used -= a["bytes"]
+                    if log:
+                        log(rec)
+                changed = True
+        if batch is not None:
+            batch.close()
+        wall = clock()
+        return done, summarize(done, wall, busy, ticks)
+
+
+def summarize(done, wall, busy=None, ticks=None):
+    tts = np.array([d["tts"] for d in done]) if done else np.zeros(1)
+    work = float(sum(d["points"] * d["steps"] for d in done))
+    out = {"requests": len(done), "wall_s": round(wall, 3),
+           "p50_s": round(float(np.percentile(tts, 50)), 4), "p99_s": round(float(np.percentile(tts, 99)), 4),
+           "mean_s": round(float(tts.mean()), 4), "point_steps": work,
+           "aggregate_gpt_steps_per_s": round(work / max(wall, 1e-9) / 1e9, 4)}
+    if busy is not None:
+        out["gpu_busy_s"] = round(busy, 3)
+        out["busy_gpt_steps_per_s"] = round(work / max(busy, 1e-9) / 1e9, 4)
+        out["ticks"] = ticks
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser(description="seeded serving workload on one GPU")
+    ap.add_argument("--requests", type=int, default=1000)
+    ap.add_argument("--rate", type=float, default=20.0, help="mean arrivals per second")
+    ap.add_argument("--seed", type=int, default=5)
+    ap.add_argument("--budget-gb", type=float, default=200.0)
+    ap.add_argument("--tick", type=int, default=10)
+    ap.add_argument("--no-realtime", action="store_true")
+    a = ap.parse_args()
+    reqs = serving(n_requests=a.requests, seed=a.seed, rate=a.rate)
+    done, summary = Server(budget_bytes=a.budget_gb * 1e9, tick_steps=a.tick).run(reqs, realtime=not a.no_realtime)
+    print(json.dumps(summary), flush=True)
+
+
+if __name__ == "__main__":
+    main()

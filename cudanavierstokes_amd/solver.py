@@ -203,8 +203,8 @@ class Solver:
 class Batch:
     """Many solvers advanced together (cns_batch_*): one launch per RK stage
     per kernel-template group, one captured graph per step.  A member's
-    result equals its own cns_step result bitwise; members cannot be stepped
-    alone while the batch exists."""
+    result equals its own cns_step result to rounding; members cannot be
+    stepped alone while the batch exists."""
 
     def __init__(self, solvers, tune=False):
         self.L = _lib.load()

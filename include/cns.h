@@ -144,8 +144,10 @@ int cns_device_bytes(const cns_solver *s, uint64_t *bytes);
  * launch per RK stage covering every member's tiles (plus one dilatation
  * launch per stage and four small control launches per step).  A step of the
  * batch is captured once as a hipGraph and replayed.  Every member follows
- * exactly the arithmetic of cns_step on its own, so a member's result after
- * n batched steps equals its result after cns_step(n) bitwise.  Per member,
+ * the arithmetic of cns_step on its own (same tiles, operators and step
+ * control); the batched kernels are compiled separately and fuse a few
+ * multiply-adds differently, so a member's result after n batched steps
+ * equals its result after cns_step(n) to rounding.  Per member,
  * steps are counted from its last cns_set_state or cns_step, across
  * successive batches (the CFL check and forcing controller run on steps 0,
  * check_cfl, 2 check_cfl, ...), so a member may move between batches.

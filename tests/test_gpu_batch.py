@@ -1,19 +1,35 @@
 """Batched advance (cns_batch_*) against each member advanced on its own.
 
-A batch runs every member through exactly the arithmetic of cns_step (same
-tiles, same kernels, same control), so after n steps each member's state,
-dt, dp/dz and time must equal the lone run's bitwise.  Members of different
-sizes, schemes, viscosity laws and boundary kinds form several groups; the
-steps are split over several cns_batch_step calls so that the captured graph
-is replayed with host work in between.
+A batch runs every member through the arithmetic of cns_step (same tiles,
+same operators, same step control).  The batched kernel is compiled
+separately from the single-instance one and the compiler fuses a few
+multiply-adds differently, so results agree to rounding rather than bitwise:
+relative L2 <= 1e-12 and Linf <= 1e-10 per conserved variable for FP64
+storage, 1e-6 / 1e-5 for FP32 storage (scalars dt, dp/dz, time: relative
+1e-12 / 1e-6).  Members of different sizes, schemes, viscosity laws and
+boundary kinds form several groups; the steps are split over several
+cns_batch_step calls so that the captured graph is replayed with host work
+in between.
 """
 import numpy as np
 import pytest
 
 import cudanavierstokes_amd as cns
-from tests.helpers import flush_print, initial_fields, oracle_from
+from tests.helpers import assert_close, flush_print, initial_fields, oracle_from, state_errors
 
 pytestmark = pytest.mark.gpu
+
+TOL = {0: (1e-12, 1e-10, 1e-12), 1: (1e-6, 1e-5, 1e-6)}  # precision -> L2, Linf, scalars
+
+
+def _same(name, precision, got, want, sc_got=None, sc_want=None):
+    l2, linf, rs = TOL[precision]
+    errs = state_errors(got, want)
+    assert_close(errs, l2, linf, name)
+    if sc_got is not None:
+        for k in ("dt", "dpdz", "time"):
+            assert sc_got[k] == pytest.approx(sc_want[k], rel=rs, abs=1e-300), (name, k)
+
 
 CHANNEL = dict(mx=24, my=16, mz=20, stencil_size=3, stencil_visc=3, periodic_x=0, nonuniform_x=1, forcing=1,
                Lx=2.0, Ly=4 * np.pi / 3, Lz=4 * np.pi, Re=3000.0, Pr=0.72, Ma=1.5, viscexp=0.7, scheme=0,
@@ -57,10 +73,7 @@ def _compare(precision, splits, tune=False):
         b.step(k)
     b.close()
     for (name, _, _), s, (st_a, sc_a) in zip(MEMBERS, batch_members, alone):
-        st_b = s.get_state()
-        for q in range(5):
-            assert np.array_equal(st_a[q], st_b[q]), f"{name}: field {q} differs"
-        assert s.scalars() == sc_a, name
+        _same(name, precision, s.get_state(), st_a, s.scalars(), sc_a)
         s.close()
 
 
@@ -91,9 +104,7 @@ def test_members_move_between_batches():
         b2.step(2)
         b2.step(2)
     for (name, _, _), s, w in zip(sel, (a, b, c), want):
-        got = s.get_state()
-        for q in range(5):
-            assert np.array_equal(got[q], w[q]), f"{name}: field {q} differs"
+        _same(name, 0, s.get_state(), w)
         s.close()
 
 
