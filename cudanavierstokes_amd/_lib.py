@@ -43,7 +43,7 @@ EXPORTS = [
     "cns_fill_taylor_green", "cns_fill_channel", "cns_set_state", "cns_get_state", "cns_rhs",
     "cns_step", "cns_set_sponge", "cns_get_scalars", "cns_sync", "cns_profile_enable",
     "cns_profile_read", "cns_time_steps", "cns_device_bytes",
-    "cns_batch_create", "cns_batch_tune", "cns_batch_step", "cns_batch_time_steps", "cns_batch_group_info",
+    "cns_batch_create", "cns_batch_tune", "cns_batch_step", "cns_batch_launch", "cns_batch_wait", "cns_batch_time_steps", "cns_batch_group_info",
     "cns_batch_groups", "cns_batch_destroy", "cns_batch_last_error",
 ]
 
@@ -87,6 +87,8 @@ def load(path=None):
     L.cns_batch_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(vp)]
     L.cns_batch_tune.argtypes = [vp]
     L.cns_batch_step.argtypes = [vp, C.c_int]
+    L.cns_batch_launch.argtypes = [vp, C.c_int]
+    L.cns_batch_wait.argtypes = [vp]
     L.cns_batch_time_steps.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
     ip = C.POINTER(C.c_int)
     L.cns_batch_group_info.argtypes = [vp, C.c_int, ip, ip, ip, ip, ip]

@@ -19,6 +19,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/cns.h"
@@ -89,6 +90,25 @@ struct SplitMix64 {
     }
     double uniform() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
 };
+
+// Split [0, n) over the host's hardware threads (at most 16) for the
+// per-point host loops of initial conditions and FP32 packing.
+template <class F>
+void host_parallel(long n, F f) {
+    unsigned nt = std::thread::hardware_concurrency();
+    nt = nt < 1 ? 1 : (nt > 16 ? 16 : nt);
+    if (n < 65536 || nt == 1) {
+        f(0L, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    const long chunk = (n + nt - 1) / nt;
+    for (unsigned t = 0; t < nt; t++) {
+        const long a = t * chunk, b = std::min(n, a + chunk);
+        if (a < b) th.emplace_back([=]() { f(a, b); });
+    }
+    for (auto &x : th) x.join();
+}
 
 }  // namespace
 
@@ -542,9 +562,10 @@ extern "C" int cns_fill_taylor_green(const cns_solver *s, double *r, double *u, 
     const cns_config &c = s->cfg;
     const double V0 = 1.0, T0 = 1.0, P0 = T0 * s->Rgas, R0 = 1.0;
     const int mx = c.nx, my = c.ny, mz = c.nz;
-    for (int k = 0; k < mz; k++) {
-        const double fz = 2 * M_PI * s->z[k] / c.len_z;
-        for (int j = 0; j < my; j++) {
+    host_parallel((long)my * mz, [&](long a, long b) {
+        for (long jk = a; jk < b; jk++) {
+            const int j = (int)(jk % my), k = (int)(jk / my);
+            const double fz = 2 * M_PI * s->z[k] / c.len_z;
             const double fy = 2 * M_PI * s->y[j] / c.len_y;
             for (int i = 0; i < mx; i++) {
                 const double fx = 2 * M_PI * s->x[i] / c.len_x;
@@ -552,13 +573,13 @@ extern "C" int cns_fill_taylor_green(const cns_solver *s, double *r, double *u, 
                 u[g] = V0 * sin(fx / 1.0) * cos(fy / 1.0) * cos(fz / 1.0);
                 v[g] = -V0 * cos(fx / 1.0) * sin(fy / 1.0) * cos(fz / 1.0);
                 w[g] = 0.0;
-                const double press =
-                    P0 + 1.0 / 16.0 * R0 * V0 * V0 * (cos(2.0 * fx / 1.0) + cos(2.0 * fy / 1.0)) * (cos(2.0 * fz / 1.0) + 2.0);
+                const double press = P0 + 1.0 / 16.0 * R0 * V0 * V0 * (cos(2.0 * fx / 1.0) + cos(2.0 * fy / 1.0)) *
+                                              (cos(2.0 * fz / 1.0) + 2.0);
                 r[g] = press / s->Rgas / T0;
                 e[g] = press / (c.gamma - 1.0) + 0.5 * r[g] * (pow(u[g], 2) + pow(v[g], 2) + pow(w[g], 2));
             }
         }
-    }
+    });
     return CNS_OK;
 }
 
@@ -571,19 +592,24 @@ extern "C" int cns_fill_channel(const cns_solver *s, uint64_t seed, double ubulk
     if (!s || !r || !u || !v || !w || !e) return fail(nullptr, CNS_ERR_ARG, "cns_fill_channel: null argument");
     const cns_config &c = s->cfg;
     const double T0 = 1.0, P0 = T0 * s->Rgas, R0 = 1.0, hh = c.len_x / 2.0;
-    SplitMix64 rng{seed};
     const int mx = c.nx, my = c.ny, mz = c.nz;
-    for (long g = 0; g < (long)mx * my * mz; g++) {
-        const int i = (int)(g % mx);
-        const double xi = s->x[i] / hh - 1.0;
-        const double wm = 1.5 * ubulk * (1.0 - xi * xi);
-        const double a1 = rng.uniform(), a2 = rng.uniform(), a3 = rng.uniform();
-        u[g] = amp * ubulk * (2.0 * a1 - 1.0);
-        v[g] = amp * ubulk * (2.0 * a2 - 1.0);
-        w[g] = wm + amp * ubulk * (2.0 * a3 - 1.0);
-        r[g] = R0;
-        e[g] = P0 / (c.gamma - 1.0) + 0.5 * r[g] * (u[g] * u[g] + v[g] * v[g] + w[g] * w[g]);
-    }
+    // SplitMix64's state after m draws is seed + m * 0x9E3779B97F4A7C15, so
+    // point g's three draws start from seed + 3 g times that increment: the
+    // same numbers as one sequential stream, drawn in parallel
+    host_parallel((long)mx * my * mz, [&](long a, long b) {
+        SplitMix64 rng{seed + (uint64_t)(3 * a) * 0x9E3779B97F4A7C15ull};
+        for (long g = a; g < b; g++) {
+            const int i = (int)(g % mx);
+            const double xi = s->x[i] / hh - 1.0;
+            const double wm = 1.5 * ubulk * (1.0 - xi * xi);
+            const double a1 = rng.uniform(), a2 = rng.uniform(), a3 = rng.uniform();
+            u[g] = amp * ubulk * (2.0 * a1 - 1.0);
+            v[g] = amp * ubulk * (2.0 * a2 - 1.0);
+            w[g] = wm + amp * ubulk * (2.0 * a3 - 1.0);
+            r[g] = R0;
+            e[g] = P0 / (c.gamma - 1.0) + 0.5 * r[g] * (u[g] * u[g] + v[g] * v[g] + w[g] * w[g]);
+        }
+    });
     return CNS_OK;
 }
 
@@ -661,14 +687,16 @@ extern "C" int cns_set_state(cns_solver *s, const double *r, const double *u, co
         }
         set_offsets(s, sr / n, se / n);
         std::vector<float> hi(5 * (size_t)n), lo(5 * (size_t)n);
-        for (long g = 0; g < n; g++) {
-            const double x[5] = {r[g] - s->rho0, u[g] * r[g], v[g] * r[g], w[g] * r[g], e[g] - s->E0};
-            for (int q = 0; q < 5; q++) {
-                const float h = (float)x[q];
-                hi[q * n + g] = h;
-                lo[q * n + g] = (float)(x[q] - (double)h);
+        host_parallel(n, [&](long a, long b) {
+            for (long g = a; g < b; g++) {
+                const double x[5] = {r[g] - s->rho0, u[g] * r[g], v[g] * r[g], w[g] * r[g], e[g] - s->E0};
+                for (int q = 0; q < 5; q++) {
+                    const float h = (float)x[q];
+                    hi[q * n + g] = h;
+                    lo[q * n + g] = (float)(x[q] - (double)h);
+                }
             }
-        }
+        });
         HIPCHK(s, hipMemcpy(s->sets[0], hi.data(), hi.size() * sizeof(float), hipMemcpyHostToDevice));
         if (s->comp) HIPCHK(s, hipMemcpy(s->lo[0], lo.data(), lo.size() * sizeof(float), hipMemcpyHostToDevice));
     }
@@ -708,15 +736,17 @@ extern "C" int cns_get_state(cns_solver *s, double *r, double *u, double *v, dou
     HIPCHK(s, hipStreamSynchronize(s->stream));
     HIPCHK(s, hipMemcpy(hi.data(), s->sets[s->cur], hi.size() * sizeof(float), hipMemcpyDeviceToHost));
     if (s->comp) HIPCHK(s, hipMemcpy(lo.data(), s->lo[s->cur], lo.size() * sizeof(float), hipMemcpyDeviceToHost));
-    for (long g = 0; g < n; g++) {
-        double x[5];
-        for (int q = 0; q < 5; q++) x[q] = (double)hi[q * n + g] + (double)lo[q * n + g];
-        r[g] = x[0] + s->rho0;
-        u[g] = x[1] / r[g];
-        v[g] = x[2] / r[g];
-        w[g] = x[3] / r[g];
-        e[g] = x[4] + s->E0;
-    }
+    host_parallel(n, [&](long a, long b) {
+        for (long g = a; g < b; g++) {
+            double x[5];
+            for (int q = 0; q < 5; q++) x[q] = (double)hi[q * n + g] + (double)lo[q * n + g];
+            r[g] = x[0] + s->rho0;
+            u[g] = x[1] / r[g];
+            v[g] = x[2] / r[g];
+            w[g] = x[3] / r[g];
+            e[g] = x[4] + s->E0;
+        }
+    });
     return CNS_OK;
 }
 
@@ -1099,6 +1129,7 @@ struct cns_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     long steps = 0;
+    int pending = 0;  // steps launched by cns_batch_launch and not yet waited for
     bool sync_check = false;
 };
 
@@ -1391,6 +1422,10 @@ static int time_stage_any(cns_batch *b, const BatchGroup &g, float &best) {
 
 extern "C" int cns_batch_tune(cns_batch *b) {
     if (!b) return bfail(nullptr, CNS_ERR_ARG, "null batch");
+    if (b->pending) {
+        const int rc = cns_batch_wait(b);
+        if (rc) return rc;
+    }
     BCHK(b, hipSetDevice(b->dev));
     for (BatchGroup &g : b->groups) {
         int maxz = 0;
@@ -1467,6 +1502,10 @@ static void batch_after(cns_batch *b, int nsteps) {
 
 extern "C" int cns_batch_step(cns_batch *b, int nsteps) {
     if (!b) return bfail(nullptr, CNS_ERR_ARG, "null batch");
+    if (b->pending) {
+        const int rc = cns_batch_wait(b);
+        if (rc) return rc;
+    }
     if (nsteps < 0) return bfail(b, CNS_ERR_ARG, "cns_batch_step: nsteps < 0");
     BCHK(b, hipSetDevice(b->dev));
     int rc = batch_run(b, nsteps);
@@ -1476,8 +1515,32 @@ extern "C" int cns_batch_step(cns_batch *b, int nsteps) {
     return CNS_OK;
 }
 
+extern "C" int cns_batch_launch(cns_batch *b, int nsteps) {
+    if (!b) return bfail(nullptr, CNS_ERR_ARG, "null batch");
+    if (nsteps < 0) return bfail(b, CNS_ERR_ARG, "cns_batch_launch: nsteps < 0");
+    BCHK(b, hipSetDevice(b->dev));
+    int rc = batch_run(b, nsteps);
+    if (rc) return rc;
+    b->pending += nsteps;
+    return CNS_OK;
+}
+
+extern "C" int cns_batch_wait(cns_batch *b) {
+    if (!b) return bfail(nullptr, CNS_ERR_ARG, "null batch");
+    BCHK(b, hipSetDevice(b->dev));
+    BCHK(b, hipStreamSynchronize(b->stream));
+    const int n = b->pending;
+    b->pending = 0;
+    batch_after(b, n);
+    return CNS_OK;
+}
+
 extern "C" int cns_batch_time_steps(cns_batch *b, int nsteps, double *ms) {
     if (!b || !ms) return bfail(b, CNS_ERR_ARG, "cns_batch_time_steps: null argument");
+    if (b->pending) {
+        const int rc = cns_batch_wait(b);
+        if (rc) return rc;
+    }
     if (nsteps < 0) return bfail(b, CNS_ERR_ARG, "cns_batch_time_steps: nsteps < 0");
     BCHK(b, hipSetDevice(b->dev));
     hipEvent_t e0, e1;
@@ -1516,6 +1579,8 @@ extern "C" int cns_batch_destroy(cns_batch *b) {
     if (!b) return CNS_OK;
     (void)hipSetDevice(b->dev);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->pending) batch_after(b, b->pending);
+    b->pending = 0;
     drop_graph(b);
     for (BatchGroup &g : b->groups)
         if (g.d_jobs) (void)hipFree(g.d_jobs);

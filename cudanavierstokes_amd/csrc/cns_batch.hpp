@@ -39,6 +39,29 @@ struct Job {
     int nwork;     // work items (tiles x chunks) of this instance
 };
 
+// A pointer read from device memory is generic to the compiler; rebuilt as
+// an offset from a kernel-argument pointer (which the compiler knows to be
+// global) its loads and stores use global instead of flat instructions.
+template <typename T>
+__device__ __forceinline__ T *glob(const void *base, T *p) {
+    const char *b = static_cast<const char *>(base);
+    return (T *)(b + ((const char *)p - b));
+}
+
+// The stencil loops address everything from qin and dil, so those two are
+// rebuilt.  Done for FP64 storage only: in the FP32 kernel (two waves per
+// SIMD, at most 256 VGPRs) the extra 64-bit address arithmetic spills, which
+// costs more than the flat loads it removes.
+template <typename R>
+__device__ __forceinline__ StagePtrs<R> stage_ptrs(const void *base, const StagePtrs<R> &a) {
+    StagePtrs<R> g = a;
+    if (sizeof(R) == 8) {
+        g.qin = glob(base, a.qin);
+        g.dil = glob(base, a.dil);
+    }
+    return g;
+}
+
 // index of the job that owns work item w (jobs sorted by first)
 template <typename R>
 __device__ __forceinline__ int job_of(const Job<R> *__restrict__ jobs, int njobs, int w) {
@@ -65,7 +88,7 @@ __global__ __launch_bounds__(TX *TY, (rhs_waves<R, UNITV>())) void k_rhs_batch(c
         const int jb = job_of(jobs, njobs, w);
         const Job<R> &J = jobs[jb];
         const int par = J.ctl->istep & 1;
-        rhs_tile<R, S, SV, TX, TY, WALLX, NONUNI, UNITV>(J.P, J.A[par], w - J.first);
+        rhs_tile<R, S, SV, TX, TY, WALLX, NONUNI, UNITV>(J.P, stage_ptrs(jobs, J.A[par]), w - J.first);
         __syncthreads();  // LDS reuse by the next item
     }
 }
@@ -78,7 +101,7 @@ __global__ __launch_bounds__(TX *TY) void k_dil_batch(const Job<R> *__restrict__
         const int jb = job_of(jobs, njobs, w);
         const Job<R> &J = jobs[jb];
         const int par = J.ctl->istep & 1;
-        dil_tile<R, SV, TX, TY, WALLX, NONUNI>(J.P, J.A[par], J.dil, w - J.first);
+        dil_tile<R, SV, TX, TY, WALLX, NONUNI>(J.P, stage_ptrs(jobs, J.A[par]), glob(jobs, J.dil), w - J.first);
         __syncthreads();
     }
 }

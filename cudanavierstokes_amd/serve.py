@@ -54,33 +54,42 @@ class Server:
         per request and the run's summary."""
         pending = collections.deque(sorted(requests, key=lambda r: r["arrival"]))
         waiting = collections.deque()
-        active = {}
+        active = {}    # admitted and in the current batch
+        joining = {}   # admitted while the batch runs; they join at the next tick
         done = []
-        used = 0.0
+        used = [0.0]
         batch = None
-        changed = False
-        shift = 0.0
+        shift = [0.0]
         t0 = time.perf_counter()
-        clock = lambda: time.perf_counter() - t0 + shift  # noqa: E731
+        clock = lambda: time.perf_counter() - t0 + shift[0]  # noqa: E731
         busy = 0.0
         ticks = 0
-        while pending or waiting or active:
+
+        def admit():
             now = clock()
             while pending and pending[0]["arrival"] <= now:
                 waiting.append(pending.popleft())
-            while waiting and len(active) < self.max_members:
+            while waiting and len(active) + len(joining) < self.max_members:
                 r = waiting[0]
                 need = footprint(r["config"])
                 if need > self.budget:
                     raise ValueError(f"request {r['id']} needs {need:.3g} B, above the budget")
-                if used + need > self.budget:
+                if used[0] + need > self.budget:
                     break
                 waiting.popleft()
                 s = Solver(r["config"])
                 s.set_state(*fields_for(s, r["kind"], r.get("seed", 1)))
-                active[r["id"]] = dict(req=r, solver=s, left=int(r["steps"]), admitted=clock(), bytes=need)
-                used += need
-                changed = True
+                joining[r["id"]] = dict(req=r, solver=s, left=int(r["steps"]), admitted=clock(), bytes=need)
+                used[0] += need
+
+        while pending or waiting or active or joining:
+            admit()
+            if joining:
+                active.update(joining)
+                joining.clear()
+                if batch is not None:
+                    batch.close()
+                    batch = None
             if not active:
                 if pending:
                     wait = pending[0]["arrival"] - clock()
@@ -88,16 +97,15 @@ class Server:
                         if realtime:
                             time.sleep(wait)
                         else:
-                            shift += wait
+                            shift[0] += wait
                 continue
-            if changed or batch is None:
-                if batch is not None:
-                    batch.close()
+            if batch is None:
                 batch = Batch([a["solver"] for a in active.values()], tune=self.tune)
-                changed = False
             k = min(self.tick, min(a["left"] for a in active.values()))
             tb = time.perf_counter()
-            batch.step(k)
+            batch.launch(k)
+            admit()  # host work for newcomers overlaps the batch on the GPU
+            batch.wait()
             busy += time.perf_counter() - tb
             ticks += 1
             finished = []
@@ -120,10 +128,9 @@ class Server:
                         rec["state"] = state
                     done.append(rec)
                     a["solver"].close()
-                    used -= a["bytes"]
+                    used[0] -= a["bytes"]
                     if log:
                         log(rec)
-                changed = True
         if batch is not None:
             batch.close()
         wall = clock()
@@ -138,8 +145,8 @@ def summarize(done, wall, busy=None, ticks=None):
            "mean_s": round(float(tts.mean()), 4), "point_steps": work,
            "aggregate_gpt_steps_per_s": round(work / max(wall, 1e-9) / 1e9, 4)}
     if busy is not None:
-        out["gpu_busy_s"] = round(busy, 3)
-        out["busy_gpt_steps_per_s"] = round(work / max(busy, 1e-9) / 1e9, 4)
+        out["tick_s"] = round(busy, 3)
+        out["tick_gpt_steps_per_s"] = round(work / max(busy, 1e-9) / 1e9, 4)
         out["ticks"] = ticks
     return out
 

@@ -226,6 +226,13 @@ class Batch:
     def step(self, nsteps):
         self._chk(self.L.cns_batch_step(self.h, nsteps), "cns_batch_step")
 
+    def launch(self, nsteps):
+        """queue nsteps steps and return; wait() before reading members"""
+        self._chk(self.L.cns_batch_launch(self.h, nsteps), "cns_batch_launch")
+
+    def wait(self):
+        self._chk(self.L.cns_batch_wait(self.h), "cns_batch_wait")
+
     def time_steps(self, nsteps):
         ms = C.c_double()
         self._chk(self.L.cns_batch_time_steps(self.h, nsteps, C.byref(ms)), "cns_batch_time_steps")

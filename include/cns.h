@@ -161,6 +161,12 @@ int cns_batch_create(cns_solver *const *members, int n_members, cns_batch **out)
 int cns_batch_tune(cns_batch *b);
 /* n_steps steps of every member (graph replays on the batch stream) */
 int cns_batch_step(cns_batch *b, int n_steps);
+/* the same without waiting: cns_batch_launch queues n_steps and returns;
+ * cns_batch_wait waits for everything queued.  Members must not be read
+ * before the wait (cns_batch_step, _tune, _time_steps and _destroy wait
+ * first). */
+int cns_batch_launch(cns_batch *b, int n_steps);
+int cns_batch_wait(cns_batch *b);
 /* the same, timed with HIP events on the batch stream */
 int cns_batch_time_steps(cns_batch *b, int n_steps, double *ms);
 /* tuned launch shape of group g: pencil length, workgroups, XCD ordering, work items */
