@@ -45,15 +45,26 @@ __device__ __forceinline__ unsigned src_offset(const Params &P, int gi, int gj, 
     return (unsigned)(gi + P.mx * gj) * esz;
 }
 
-// Loads address a uniform plane base plus a 32-bit per-lane byte offset, the
-// form of a global load with a scalar base.
-template <typename R>
-__device__ __forceinline__ R ld_at(const R *base, unsigned boff) {
-    return *reinterpret_cast<const R *>(reinterpret_cast<const char *>(base) + boff);
+// Field accesses are raw buffer loads and stores: a uniform plane base in a
+// buffer resource (scalar registers, rebuilt by scalar instructions per
+// plane) plus a 32-bit per-lane byte offset, so no 64-bit address arithmetic
+// runs on the vector ALU.  The resource covers 2^32 bytes from the base with
+// no stride (third word 0x00020000 for gfx9 raw buffers).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc_of(const void *base) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, -1, 0x00020000);
 }
-template <typename R>
-__device__ __forceinline__ void st_at(R *base, unsigned boff, R v) {
-    *reinterpret_cast<R *>(reinterpret_cast<char *>(base) + boff) = v;
+using u32x2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(rsrc_of(nullptr), 0, 0, 0));
+__device__ __forceinline__ double ld_at(const double *base, unsigned boff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc_of(base), (int)boff, 0, 0));
+}
+__device__ __forceinline__ float ld_at(const float *base, unsigned boff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsrc_of(base), (int)boff, 0, 0));
+}
+__device__ __forceinline__ void st_at(double *base, unsigned boff, double v) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), rsrc_of(base), (int)boff, 0, 0);
+}
+__device__ __forceinline__ void st_at(float *base, unsigned boff, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), rsrc_of(base), (int)boff, 0, 0);
 }
 
 // Conserved values and dilatation of one point, as loaded.

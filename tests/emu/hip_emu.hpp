@@ -38,6 +38,36 @@ inline double emu_shfl[1024];
 
 inline void __syncthreads();
 
+// raw buffer accesses: a resource is its base address; the offsets are the
+// unsigned 32-bit byte offsets the hardware adds to it
+struct __amdgpu_buffer_rsrc_t {
+    char *base = nullptr;
+};
+struct emu_u32x2 {
+    unsigned x, y;
+};
+inline __amdgpu_buffer_rsrc_t __builtin_amdgcn_make_buffer_rsrc(void *p, short, int, int) {
+    __amdgpu_buffer_rsrc_t r;
+    r.base = static_cast<char *>(p);
+    return r;
+}
+inline emu_u32x2 __builtin_amdgcn_raw_buffer_load_b64(__amdgpu_buffer_rsrc_t r, int vo, int so, int) {
+    emu_u32x2 v;
+    std::memcpy(&v, r.base + (size_t)(unsigned)vo + (size_t)(unsigned)so, 8);
+    return v;
+}
+inline unsigned __builtin_amdgcn_raw_buffer_load_b32(__amdgpu_buffer_rsrc_t r, int vo, int so, int) {
+    unsigned v;
+    std::memcpy(&v, r.base + (size_t)(unsigned)vo + (size_t)(unsigned)so, 4);
+    return v;
+}
+inline void __builtin_amdgcn_raw_buffer_store_b64(emu_u32x2 v, __amdgpu_buffer_rsrc_t r, int vo, int so, int) {
+    std::memcpy(r.base + (size_t)(unsigned)vo + (size_t)(unsigned)so, &v, 8);
+}
+inline void __builtin_amdgcn_raw_buffer_store_b32(unsigned v, __amdgpu_buffer_rsrc_t r, int vo, int so, int) {
+    std::memcpy(r.base + (size_t)(unsigned)vo + (size_t)(unsigned)so, &v, 4);
+}
+
 // scheduling hints and wave-uniform reads have no effect on the CPU
 #define __builtin_amdgcn_sched_barrier(m) ((void)(m))
 inline int __builtin_amdgcn_readfirstlane(int v) { return v; }
