@@ -1415,9 +1415,14 @@ static int time_stage(cns_batch *b, const BatchGroup &g, int st, int reps, float
     return rc;
 }
 
+// Cost of a launch shape: best of five runs of the middle stage (after one
+// unmeasured run that absorbs first-launch and cache effects).
 static int time_stage_any(cns_batch *b, const BatchGroup &g, float &best) {
     const int st = g.nst / 2;
-    return g.fp32 ? time_stage<float>(b, g, st, 3, best) : time_stage<double>(b, g, st, 3, best);
+    float warm;
+    int rc = g.fp32 ? time_stage<float>(b, g, st, 1, warm) : time_stage<double>(b, g, st, 1, warm);
+    if (rc) return rc;
+    return g.fp32 ? time_stage<float>(b, g, st, 5, best) : time_stage<double>(b, g, st, 5, best);
 }
 
 extern "C" int cns_batch_tune(cns_batch *b) {
@@ -1449,22 +1454,27 @@ extern "C" int cns_batch_tune(cns_batch *b) {
                 bestL = L;
             }
         }
-        g.pencil = bestL;
-        int rc = upload(b, g, g.pencil, true);
-        if (rc) return rc;
-        // 2. persistent workgroups (multiples of the CU count keep whole XCDs busy)
+        // 2. persistent workgroups (multiples of the CU count keep whole XCDs
+        //    busy), jointly with the best pencil length, its half and double
         int bestG = g.grid;
         bt = 1e30f;
-        for (int f : {1, 2, 4, 8}) {
-            g.grid = f * b->cus;
-            float t;
-            if ((rc = time_stage_any(b, g, t))) return rc;
-            if (t < bt) {
-                bt = t;
-                bestG = g.grid;
+        int rc = CNS_OK;
+        for (int L : pencils) {
+            if (L != bestL && L != 2 * bestL && 2 * L != bestL) continue;
+            if ((rc = upload(b, g, L, true))) return rc;
+            for (int f : {1, 2, 4, 8}) {
+                g.grid = f * b->cus;
+                float t;
+                if ((rc = time_stage_any(b, g, t))) return rc;
+                if (t < bt) {
+                    bt = t;
+                    bestG = g.grid;
+                    g.pencil = L;
+                }
             }
         }
         g.grid = bestG;
+        if ((rc = upload(b, g, g.pencil, true))) return rc;
         // 3. XCD-contiguous item order on or off
         float t0, t1;
         g.remap = 0;

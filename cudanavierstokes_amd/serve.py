@@ -64,6 +64,7 @@ class Server:
         clock = lambda: time.perf_counter() - t0 + shift[0]  # noqa: E731
         busy = 0.0
         ticks = 0
+        cost = collections.Counter()  # host seconds per phase
 
         def admit():
             now = clock()
@@ -77,8 +78,10 @@ class Server:
                 if used[0] + need > self.budget:
                     break
                 waiting.popleft()
+                ta = time.perf_counter()
                 s = Solver(r["config"])
                 s.set_state(*fields_for(s, r["kind"], r.get("seed", 1)))
+                cost["admit_s"] += time.perf_counter() - ta
                 joining[r["id"]] = dict(req=r, solver=s, left=int(r["steps"]), admitted=clock(), bytes=need)
                 used[0] += need
 
@@ -100,7 +103,9 @@ class Server:
                             shift[0] += wait
                 continue
             if batch is None:
+                tb = time.perf_counter()
                 batch = Batch([a["solver"] for a in active.values()], tune=self.tune)
+                cost["batch_build_s"] += time.perf_counter() - tb
             k = min(self.tick, min(a["left"] for a in active.values()))
             tb = time.perf_counter()
             batch.launch(k)
@@ -114,6 +119,7 @@ class Server:
                 if a["left"] == 0:
                     finished.append(rid)
             if finished:
+                tf = time.perf_counter()
                 batch.close()
                 batch = None
                 for rid in finished:
@@ -131,10 +137,13 @@ class Server:
                     used[0] -= a["bytes"]
                     if log:
                         log(rec)
+                cost["finish_s"] += time.perf_counter() - tf
         if batch is not None:
             batch.close()
         wall = clock()
-        return done, summarize(done, wall, busy, ticks)
+        out = summarize(done, wall, busy, ticks)
+        out.update({k: round(v, 3) for k, v in cost.items()})
+        return done, out
 
 
 def summarize(done, wall, busy=None, ticks=None):
