@@ -40,7 +40,7 @@ C_NAMES = {"mx": "nx", "my": "ny", "mz": "nz", "forcing": "forcing_z", "Lx": "le
 
 EXPORTS = [
     "cns_config_init", "cns_create", "cns_destroy", "cns_last_error", "cns_get_grid",
-    "cns_fill_taylor_green", "cns_fill_channel", "cns_set_state", "cns_get_state", "cns_rhs",
+    "cns_fill_taylor_green", "cns_fill_channel", "cns_init_taylor_green", "cns_init_channel", "cns_set_state", "cns_get_state", "cns_rhs",
     "cns_step", "cns_set_sponge", "cns_get_scalars", "cns_sync", "cns_profile_enable",
     "cns_profile_read", "cns_time_steps", "cns_device_bytes",
     "cns_batch_create", "cns_batch_tune", "cns_batch_step", "cns_batch_launch", "cns_batch_wait", "cns_batch_time_steps", "cns_batch_group_info",
@@ -73,6 +73,8 @@ def load(path=None):
     L.cns_get_grid.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     L.cns_fill_taylor_green.argtypes = [vp, dp, dp, dp, dp, dp]
     L.cns_fill_channel.argtypes = [vp, C.c_uint64, C.c_double, C.c_double, dp, dp, dp, dp, dp]
+    L.cns_init_taylor_green.argtypes = [vp]
+    L.cns_init_channel.argtypes = [vp, C.c_uint64, C.c_double, C.c_double]
     L.cns_set_state.argtypes = [vp, dp, dp, dp, dp, dp]
     L.cns_get_state.argtypes = [vp, dp, dp, dp, dp, dp]
     L.cns_rhs.argtypes = [vp, dp, dp, dp, dp, dp]

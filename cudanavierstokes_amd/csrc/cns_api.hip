@@ -125,6 +125,7 @@ struct cns_solver {
     double cF[4] = {0}, cS[5] = {0}, cVF[4] = {0}, cVS[5] = {0};
     // device
     double *d_xp = nullptr, *d_vsx = nullptr, *d_dxv = nullptr, *d_spx = nullptr, *d_spz = nullptr;
+    double *d_x = nullptr, *d_y = nullptr, *d_z = nullptr;  // coordinates (device initial conditions)
     double *d_ref = nullptr;  // 5 sponge reference fields, mx*mz each
     Scalars *d_sc = nullptr;
     StepCtl *d_ctl = nullptr;  // per-member step control when advanced in a batch
@@ -417,6 +418,9 @@ extern "C" int cns_destroy(cns_solver *s) {
         if (p) (void)hipFree(p);
     };
     fr(s->d_xp);
+    fr(s->d_x);
+    fr(s->d_y);
+    fr(s->d_z);
     fr(s->d_vsx);
     fr(s->d_dxv);
     fr(s->d_spx);
@@ -492,6 +496,8 @@ extern "C" int cns_create(const cns_config *cfg, cns_solver **out) {
     const long n = s->n;
     s->nsets = (cfg->scheme == CNS_RK3_LOWSTORAGE) ? 2 : (cfg->scheme == CNS_RK3_KUTTA ? 3 : 4);
     s->nrhs = (cfg->scheme == CNS_RK4) ? 1 : 2;
+    if ((rc = dalloc(s, &s->d_x, mx)) || (rc = dalloc(s, &s->d_y, cfg->ny)) || (rc = dalloc(s, &s->d_z, mz)))
+        return bail(rc);
     if ((rc = dalloc(s, &s->d_xp, mx)) || (rc = dalloc(s, &s->d_vsx, s->vsx.size())) ||
         (rc = dalloc(s, &s->d_dxv, mx)) || (rc = dalloc(s, &s->d_spx, mx)) || (rc = dalloc(s, &s->d_spz, mz)) ||
         (rc = dalloc(s, &s->d_part, (size_t)NPART * RED_BLOCKS)) || (rc = balloc(s, &s->d_dil, n * s->esz)))
@@ -521,6 +527,9 @@ extern "C" int cns_create(const cns_config *cfg, cns_solver **out) {
     h.time = 0.0;
     if (hipMemcpy(s->d_sc, &h, sizeof(h), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(s->d_xp, s->xp.data(), mx * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->d_x, s->x.data(), mx * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->d_y, s->y.data(), cfg->ny * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->d_z, s->z.data(), mz * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(s->d_vsx, s->vsx.data(), s->vsx.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(s->d_dxv, s->dxv.data(), mx * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
         s->err = "initial host-to-device copies failed";
@@ -708,6 +717,70 @@ extern "C" int cns_set_state(cns_solver *s, const double *r, const double *u, co
     s->visc_from_state = true;
     s->have_state = true;
     return CNS_OK;
+}
+
+// initial condition evaluated on the device (kind 0 Taylor-Green, 1 channel)
+static int init_on_device(cns_solver *s, int kind, uint64_t seed, double ubulk, double amp) {
+    if (s->in_batch) return fail(s, CNS_ERR_STATE, "cns_init: the solver is advanced by a batch");
+    HIPCHK(s, hipSetDevice(s->dev));
+    const cns_config &c = s->cfg;
+    InitArgs a{};
+    a.kind = kind;
+    a.mx = c.nx;
+    a.my = c.ny;
+    a.mz = c.nz;
+    a.x = s->d_x;
+    a.y = s->d_y;
+    a.z = s->d_z;
+    a.lx = c.len_x;
+    a.ly = c.len_y;
+    a.lz = c.len_z;
+    a.Rgas = s->Rgas;
+    a.gam = c.gamma;
+    a.ubulk = ubulk;
+    a.amp = amp;
+    a.seed = seed;
+    const long n = s->n;
+    s->cur = 0;
+    if (!s->fp32) {
+        hipLaunchKernelGGL(k_init_store<double>, dim3(1024), dim3(256), 0, s->stream, a, n, as<double>(s->sets[0]),
+                           (double *)nullptr, 0.0, 0.0);
+        LAUNCHED(s, "k_init_store<double>");
+    } else {
+        // reference state: the mean density and total energy, summed in a fixed order
+        hipLaunchKernelGGL(k_init_sums, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s->stream, a, n, s->d_part);
+        LAUNCHED(s, "k_init_sums");
+        std::vector<double> part(2 * RED_BLOCKS);
+        HIPCHK(s, hipMemcpyAsync(part.data(), s->d_part, part.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                 s->stream));
+        HIPCHK(s, hipStreamSynchronize(s->stream));
+        double sr = 0.0, se = 0.0;
+        for (int b = 0; b < RED_BLOCKS; b++) {
+            sr += part[b];
+            se += part[RED_BLOCKS + b];
+        }
+        set_offsets(s, sr / n, se / n);
+        hipLaunchKernelGGL(k_init_store<float>, dim3(1024), dim3(256), 0, s->stream, a, n, as<float>(s->sets[0]),
+                           s->comp ? as<float>(s->lo[0]) : (float *)nullptr, s->rho0, s->E0);
+        LAUNCHED(s, "k_init_store<float>");
+    }
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(1), 0, s->stream, &s->d_sc->time, 1L, 0.0);
+    LAUNCHED(s, "k_fill");
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    s->bsteps = 0;
+    s->visc_from_state = true;
+    s->have_state = true;
+    return CNS_OK;
+}
+
+extern "C" int cns_init_taylor_green(cns_solver *s) {
+    if (!s) return fail(nullptr, CNS_ERR_ARG, "null solver");
+    return init_on_device(s, 0, 0, 0.0, 0.0);
+}
+
+extern "C" int cns_init_channel(cns_solver *s, uint64_t seed, double ubulk, double amp) {
+    if (!s) return fail(nullptr, CNS_ERR_ARG, "null solver");
+    return init_on_device(s, 1, seed, ubulk, amp);
 }
 
 extern "C" int cns_get_state(cns_solver *s, double *r, double *u, double *v, double *w, double *e) {

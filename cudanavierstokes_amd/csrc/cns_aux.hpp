@@ -212,4 +212,102 @@ static __global__ void k_fill(double *a, long n, double v) {
     for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x) a[g] = v;
 }
 
+// Initial conditions evaluated on the device: the closed forms of the host
+// fills (Taylor-Green vortex of src/init.cpp:126-148; the seeded channel),
+// written straight into a state set.
+struct InitArgs {
+    int kind;  // 0 Taylor-Green, 1 channel
+    int mx, my, mz;
+    const double *x, *y, *z;
+    double lx, ly, lz, Rgas, gam, ubulk, amp;
+    unsigned long long seed;
+};
+
+__device__ __forceinline__ double splitmix_uniform(unsigned long long state) {
+    unsigned long long z = state;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z = z ^ (z >> 31);
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// conserved (rho, rho u, rho v, rho w, rho E) of point g
+__device__ __forceinline__ void init_point(const InitArgs &a, long g, double q[5]) {
+    const int i = (int)(g % a.mx);
+    double r, u, v, w, e;
+    if (a.kind == 0) {
+        const long jk = g / a.mx;
+        const int j = (int)(jk % a.my), k = (int)(jk / a.my);
+        const double fx = 2 * M_PI * a.x[i] / a.lx, fy = 2 * M_PI * a.y[j] / a.ly, fz = 2 * M_PI * a.z[k] / a.lz;
+        u = sin(fx) * cos(fy) * cos(fz);
+        v = -cos(fx) * sin(fy) * cos(fz);
+        w = 0.0;
+        const double press = a.Rgas + 1.0 / 16.0 * (cos(2.0 * fx) + cos(2.0 * fy)) * (cos(2.0 * fz) + 2.0);
+        r = press / a.Rgas;
+        e = press / (a.gam - 1.0) + 0.5 * r * (u * u + v * v + w * w);
+    } else {
+        // draws 3g, 3g+1, 3g+2 of the SplitMix64 stream: states seed + (3g+1+d) * increment
+        const unsigned long long inc = 0x9E3779B97F4A7C15ull, base = a.seed + (unsigned long long)(3 * g) * inc;
+        const double a1 = splitmix_uniform(base + inc), a2 = splitmix_uniform(base + 2 * inc),
+                     a3 = splitmix_uniform(base + 3 * inc);
+        const double xi = a.x[i] / (a.lx / 2.0) - 1.0;
+        const double wm = 1.5 * a.ubulk * (1.0 - xi * xi);
+        u = a.amp * a.ubulk * (2.0 * a1 - 1.0);
+        v = a.amp * a.ubulk * (2.0 * a2 - 1.0);
+        w = wm + a.amp * a.ubulk * (2.0 * a3 - 1.0);
+        r = 1.0;
+        e = a.Rgas / (a.gam - 1.0) + 0.5 * r * (u * u + v * v + w * w);
+    }
+    q[0] = r;
+    q[1] = u * r;
+    q[2] = v * r;
+    q[3] = w * r;
+    q[4] = e;
+}
+
+// per-block sums of rho and rho E (fixed RED_BLOCKS partials), for the FP32
+// reference state
+static __global__ void k_init_sums(InitArgs a, long n, double *part) {
+    double sr = 0.0, se = 0.0;
+    for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x) {
+        double q[5];
+        init_point(a, g, q);
+        sr += q[0];
+        se += q[4];
+    }
+    __shared__ double sh[2][RED_THREADS];
+    sh[0][threadIdx.x] = sr;
+    sh[1][threadIdx.x] = se;
+    __syncthreads();
+    for (int size = RED_THREADS / 2; size > 0; size /= 2) {
+        if ((int)threadIdx.x < size) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + size];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + size];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = sh[0][0];
+        part[RED_BLOCKS + blockIdx.x] = sh[1][0];
+    }
+}
+
+// store the state: FP64 as is; FP32 as offsets from (rho0, E0) split into
+// hi = fl(x) and lo = fl(x - hi) (lo may be null)
+template <typename R>
+__global__ void k_init_store(InitArgs a, long n, R *set, R *lo, double rho0, double E0) {
+    for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < n; g += (long)gridDim.x * blockDim.x) {
+        double q[5];
+        init_point(a, g, q);
+        q[0] -= rho0;
+        q[4] -= E0;
+#pragma unroll
+        for (int c = 0; c < 5; c++) {
+            const R h = (R)q[c];
+            set[g + c * n] = h;
+            if (lo) lo[g + c * n] = (R)(q[c] - (double)h);
+        }
+    }
+}
+
 }  // namespace cns

@@ -22,7 +22,7 @@ import time
 import numpy as np
 
 from .solver import Batch, Solver
-from .workloads import fields_for, points, serving
+from .workloads import init_on_device, points, serving
 
 _SETS = {0: (2, 2, 2), 1: (3, 1, 2), 2: (4, 2, 1)}  # scheme -> (state sets, compensated sets, rhs sets)
 
@@ -80,7 +80,7 @@ class Server:
                 waiting.popleft()
                 ta = time.perf_counter()
                 s = Solver(r["config"])
-                s.set_state(*fields_for(s, r["kind"], r.get("seed", 1)))
+                init_on_device(s, r["kind"], r.get("seed", 1))
                 cost["admit_s"] += time.perf_counter() - ta
                 joining[r["id"]] = dict(req=r, solver=s, left=int(r["steps"]), admitted=clock(), bytes=need)
                 used[0] += need

@@ -147,6 +147,14 @@ class Solver:
         self._chk(self.L.cns_fill_channel(self.h, seed, ubulk, amp, *out), "cns_fill_channel")
         return [a.reshape(self.shape) for a in out]
 
+    def init_taylor_green(self):
+        """the Taylor-Green state computed on the device (no host arrays)"""
+        self._chk(self.L.cns_init_taylor_green(self.h), "cns_init_taylor_green")
+
+    def init_channel(self, seed=1, ubulk=1.0, amp=0.1):
+        """the seeded channel state computed on the device"""
+        self._chk(self.L.cns_init_channel(self.h, seed, ubulk, amp), "cns_init_channel")
+
     def set_state(self, r, u, v, w, e):
         self._chk(self.L.cns_set_state(self.h, *[_f64(a, self.n) for a in (r, u, v, w, e)]), "cns_set_state")
 

@@ -88,8 +88,16 @@ def fields_for(solver, kind, seed=1):
     return solver.channel_fields(seed=seed, ubulk=1.0, amp=0.1)
 
 
+def init_on_device(solver, kind, seed=1):
+    """The same initial conditions as fields_for, computed on the device."""
+    if kind == "tgv":
+        solver.init_taylor_green()
+    else:
+        solver.init_channel(seed=seed, ubulk=1.0, amp=0.1)
+
+
 def points(config):
     return config.mx * config.my * config.mz
 
 
-__all__ = ["SplitMix64", "ensemble", "serving", "fields_for", "points", "np"]
+__all__ = ["SplitMix64", "ensemble", "serving", "fields_for", "init_on_device", "points", "np"]

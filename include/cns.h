@@ -95,6 +95,12 @@ int cns_fill_taylor_green(const cns_solver *s, double *r, double *u, double *v, 
 int cns_fill_channel(const cns_solver *s, uint64_t seed, double ubulk, double amp, double *r, double *u,
                      double *v, double *w, double *e);
 
+/* The same initial conditions evaluated on the device and written straight
+ * into the state (no host arrays, no transfer); like cns_set_state they reset
+ * time.  They agree with the host fills to rounding (device sin/cos). */
+int cns_init_taylor_green(cns_solver *s);
+int cns_init_channel(cns_solver *s, uint64_t seed, double ubulk, double amp);
+
 /* replaces copyField(0): host (rho,u,v,w,rhoE) -> device; resets time */
 int cns_set_state(cns_solver *s, const double *r, const double *u, const double *v, const double *w,
                   const double *e);

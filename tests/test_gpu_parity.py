@@ -157,3 +157,24 @@ def test_initial_condition_fills():
     for a, b in zip(sol.channel_fields(seed=3), cases.channel_fields(orc.grid(), orc.cfg, seed=3)):
         assert np.array_equal(a, b)
     sol.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32"])
+def test_device_initial_conditions(precision):
+    """cns_init_* (evaluated on the device) agree with the host fills set
+    through cns_set_state: to device sin/cos rounding (Taylor-Green) and to
+    FMA rounding (channel), within the storage precision's round trip."""
+    tol = 1e-13 if precision == 0 else 1e-12
+    for kw, init, fill in (
+            (dict(mx=24, my=20, mz=18), lambda s: s.init_taylor_green(), lambda s: s.taylor_green_fields()),
+            (dict(STEP_CASES[3][2]), lambda s: s.init_channel(seed=3), lambda s: s.channel_fields(seed=3))):
+        cfg = cns.Config(**kw).replace(precision=precision)
+        a = cns.Solver(cfg)
+        init(a)
+        got = a.get_state()
+        b = cns.Solver(cfg)
+        b.set_state(*fill(b))
+        want = b.get_state()
+        assert_close(cases.rel_errors(cases.conserved(got), cases.conserved(want)), tol, 10 * tol, "device init")
+        a.close()
+        b.close()

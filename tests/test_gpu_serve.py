@@ -8,7 +8,7 @@ import pytest
 
 import cudanavierstokes_amd as cns
 from cudanavierstokes_amd.serve import Server, footprint
-from cudanavierstokes_amd.workloads import fields_for
+from cudanavierstokes_amd.workloads import init_on_device
 from tests.helpers import assert_close, state_errors
 
 pytestmark = pytest.mark.gpu
@@ -43,7 +43,7 @@ def test_served_results_equal_lone_runs():
     for r in reqs:
         s = cns.Solver(r["config"])
         assert abs(s.device_bytes() - footprint(r["config"])) < 2 ** 20
-        s.set_state(*fields_for(s, r["kind"], r["seed"]))
+        init_on_device(s, r["kind"], r["seed"])
         s.step(r["steps"], outputs=False)
         want = s.get_state()
         s.close()
