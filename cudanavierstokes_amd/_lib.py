@@ -43,7 +43,7 @@ EXPORTS = [
     "cns_fill_taylor_green", "cns_fill_channel", "cns_init_taylor_green", "cns_init_channel", "cns_set_state", "cns_get_state", "cns_rhs",
     "cns_step", "cns_set_sponge", "cns_get_scalars", "cns_sync", "cns_profile_enable",
     "cns_profile_read", "cns_time_steps", "cns_device_bytes",
-    "cns_batch_create", "cns_batch_tune", "cns_batch_step", "cns_batch_launch", "cns_batch_wait", "cns_batch_time_steps", "cns_batch_group_info",
+    "cns_batch_create", "cns_batch_tune", "cns_batch_step", "cns_batch_launch", "cns_batch_wait", "cns_batch_time_steps", "cns_batch_group_info", "cns_batch_set_shape",
     "cns_batch_groups", "cns_batch_destroy", "cns_batch_last_error",
 ]
 
@@ -95,6 +95,7 @@ def load(path=None):
     ip = C.POINTER(C.c_int)
     L.cns_batch_group_info.argtypes = [vp, C.c_int, ip, ip, ip, ip, ip]
     L.cns_batch_groups.argtypes = [vp]
+    L.cns_batch_set_shape.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
     L.cns_batch_destroy.argtypes = [vp]
     L.cns_batch_last_error.restype = C.c_char_p
     L.cns_batch_last_error.argtypes = [vp]

@@ -1644,6 +1644,24 @@ extern "C" int cns_batch_time_steps(cns_batch *b, int nsteps, double *ms) {
     return rc;
 }
 
+extern "C" int cns_batch_set_shape(cns_batch *b, int g, int pencil, int grid, int remap) {
+    if (!b) return bfail(nullptr, CNS_ERR_ARG, "null batch");
+    if (g < 0 || g >= (int)b->groups.size() || pencil < 1 || grid < 1)
+        return bfail(b, CNS_ERR_ARG, "cns_batch_set_shape: bad group, pencil or grid");
+    if (b->pending) {
+        const int rc = cns_batch_wait(b);
+        if (rc) return rc;
+    }
+    BCHK(b, hipSetDevice(b->dev));
+    BatchGroup &x = b->groups[g];
+    x.pencil = pencil;
+    x.grid = grid;
+    x.remap = remap ? 1 : 0;
+    int rc = upload(b, x, x.pencil, false);
+    if (rc) return rc;
+    return capture(b);
+}
+
 extern "C" int cns_batch_groups(const cns_batch *b) { return b ? (int)b->groups.size() : 0; }
 
 extern "C" int cns_batch_group_info(const cns_batch *b, int g, int *n_members, int *pencil, int *grid, int *remap,

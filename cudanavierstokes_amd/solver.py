@@ -246,6 +246,10 @@ class Batch:
         self._chk(self.L.cns_batch_time_steps(self.h, nsteps, C.byref(ms)), "cns_batch_time_steps")
         return ms.value
 
+    def set_shape(self, group, pencil, grid, remap=1):
+        """launch shape of one group by hand (instead of tune())"""
+        self._chk(self.L.cns_batch_set_shape(self.h, group, pencil, grid, remap), "cns_batch_set_shape")
+
     def groups(self):
         out = []
         for g in range(self.L.cns_batch_groups(self.h)):

@@ -179,6 +179,9 @@ int cns_batch_time_steps(cns_batch *b, int n_steps, double *ms);
 int cns_batch_group_info(const cns_batch *b, int g, int *n_members, int *pencil, int *grid, int *remap,
                          int *work_items);
 int cns_batch_groups(const cns_batch *b);
+/* set group g's launch shape by hand (pencil length >= 1, workgroups >= 1,
+ * XCD-contiguous order 0/1) instead of cns_batch_tune */
+int cns_batch_set_shape(cns_batch *b, int g, int pencil, int grid, int remap);
 int cns_batch_destroy(cns_batch *b);
 const char *cns_batch_last_error(const cns_batch *b);
 
