@@ -426,19 +426,22 @@ __device__ __forceinline__ void rhs_tile(const Params &P, const StagePtrs<R> &A,
     };
     const R invRe = R(P.invRe);
     const R T0 = Off<R>::on ? R(P.T0) : R(0);  // T = T0 + staged T on offset storage
-    if (act) {
+    // Every lane loads unconditionally (lanes past the edge keep the window of
+    // their ghost column, lanes without a halo point load their mirror of
+    // one, prefetches past the chunk wrap): with no loads under branches the
+    // compiler can count outstanding loads exactly and wait for just the
+    // ones it needs, so the prefetches stay in flight.
 #pragma unroll
-        for (int a = 0; a < W - 1; a++) {
-            Raw<R> x;
-            load_raw(A, (long)wrapi(z0 - H + a, P.mz) * mxy, coff, x);
-            R t[NQ];
-            to_prim<R, WALLX, UNITV>(P, x, 0, t);
-            push(t, a);
-        }
+    for (int a = 0; a < W - 1; a++) {
+        Raw<R> x;
+        load_raw(A, (long)wrapi(z0 - H + a, P.mz) * mxy, coff, x);
+        R t[NQ];
+        to_prim<R, WALLX, UNITV>(P, x, cside, t);
+        push(t, a);
     }
     Raw<R> zr, hr;  // prefetched: window feed (plane k + H) and halo point (plane k)
-    if (act) load_raw(A, (long)wrapi(z0 + H, P.mz) * mxy, coff, zr);
-    if (hact) load_raw(A, (long)z0 * mxy, hoff, hr);
+    load_raw(A, (long)wrapi(z0 + H, P.mz) * mxy, coff, zr);
+    load_raw(A, (long)z0 * mxy, hoff, hr);
     double vmax = 0.0;
     const double dtd = (P.write_q || P.write_acc) ? *P.dt : 0.0;
     const R dt = R(dtd);
@@ -449,67 +452,48 @@ __device__ __forceinline__ void rhs_tile(const Params &P, const StagePtrs<R> &A,
     for (int k = z0; k < z1; ++k) {
         const int b = (k - z0) & 1;
         const long poff = (long)k * mxy;
-        // operands of this plane's update
-        R e1[5], ea[5], eb[5], e2[5], e1l[5], e2l[5];
+        // operands of this plane's update, three slots always loaded: s1,
+        // ra, and rb (Kutta) or s2 (RK4) -- never both; arrays a stage does
+        // not use alias the stage input, so their loads are cache hits
+        R e1[5], ea[5], e3[5], e1l[5], e3l[5];
+        {
+            const R *p1 = arr(0, poff), *p2 = arr(1, poff), *p3 = arr((fl & F_RB) ? 2 : 3, poff);
 #pragma unroll
-        for (int q = 0; q < 5; q++) e1[q] = ea[q] = eb[q] = e2[q] = e1l[q] = e2l[q] = R(0);
-        if (act) {
-            if (fl & F_Q) {
-                const R *p = arr(0, poff);
-#pragma unroll
-                for (int q = 0; q < 5; q++) e1[q] = ld_at(p + q * fs, coff);
+            for (int q = 0; q < 5; q++) {
+                e1[q] = ld_at(p1 + q * fs, coff);
+                ea[q] = ld_at(p2 + q * fs, coff);
+                e3[q] = ld_at(p3 + q * fs, coff);
             }
-            if (fl & F_RA) {
-                const R *p = arr(1, poff);
+            if constexpr (Off<R>::on) {
+                const R *l1 = arr(7, poff), *l3 = arr(8, poff);
 #pragma unroll
-                for (int q = 0; q < 5; q++) ea[q] = ld_at(p + q * fs, coff);
-            }
-            if (fl & F_RB) {
-                const R *p = arr(2, poff);
-#pragma unroll
-                for (int q = 0; q < 5; q++) eb[q] = ld_at(p + q * fs, coff);
-            }
-            if (fl & F_ACC) {
-                const R *p = arr(3, poff);
-#pragma unroll
-                for (int q = 0; q < 5; q++) e2[q] = ld_at(p + q * fs, coff);
-            }
-            if (Off<R>::on && (fl & F_COMP)) {
-                if (fl & F_Q) {
-                    const R *p = arr(7, poff);
-#pragma unroll
-                    for (int q = 0; q < 5; q++) e1l[q] = ld_at(p + q * fs, coff);
+                for (int q = 0; q < 5; q++) {
+                    e1l[q] = ld_at(l1 + q * fs, coff);
+                    e3l[q] = ld_at(l3 + q * fs, coff);
                 }
-                if (fl & F_ACC) {
-                    const R *p = arr(8, poff);
+            } else {
 #pragma unroll
-                    for (int q = 0; q < 5; q++) e2l[q] = ld_at(p + q * fs, coff);
-                }
+                for (int q = 0; q < 5; q++) e1l[q] = e3l[q] = R(0);
             }
         }
         __builtin_amdgcn_sched_barrier(0);
-        if (act) {
+        {
             R t[NQ];
-            to_prim<R, WALLX, UNITV>(P, zr, 0, t);
+            to_prim<R, WALLX, UNITV>(P, zr, cside, t);
             push(t, W - 1);
 #pragma unroll
             for (int q = 0; q < NL; q++) sq[b][q][cc] = win[q][H];
-            if (k + 1 < z1) load_raw(A, (long)wrapi(k + 1 + H, P.mz) * mxy, coff, zr);
-        } else {
-            Raw<R> x;
-            load_raw(A, poff, coff, x);
-            R t[NQ];
-            to_prim<R, WALLX, UNITV>(P, x, cside, t);
-#pragma unroll
-            for (int q = 0; q < NL; q++) sq[b][q][cc] = t[q];
+            load_raw(A, (long)wrapi(k + 1 + H, P.mz) * mxy, coff, zr);
         }
         __builtin_amdgcn_sched_barrier(0);
-        if (hact) {
+        {
             R t[NQ];
             to_prim<R, WALLX, UNITV>(P, hr, hside, t);
+            if (hact) {
 #pragma unroll
-            for (int q = 0; q < NL; q++) sq[b][q][hslot] = t[q];
-            if (k + 1 < z1) load_raw(A, poff + mxy, hoff, hr);
+                for (int q = 0; q < NL; q++) sq[b][q][hslot] = t[q];
+            }
+            load_raw(A, (long)wrapi(k + 1, P.mz) * mxy, hoff, hr);
         }
         __syncthreads();
 
@@ -587,9 +571,9 @@ __device__ __forceinline__ void rhs_tile(const Params &P, const StagePtrs<R> &A,
                     for (int q = 0; q < 5; q++) {
                         double inc = 0.0;
                         if (fl & F_RA) inc += s_w[0] * (double)ea[q];
-                        if (fl & F_RB) inc += s_w[1] * (double)eb[q];
+                        if (fl & F_RB) inc += s_w[1] * (double)e3[q];
                         inc += s_w[2] * (double)rhs[q];
-                        const double x = ((double)e1[q] + (double)e1l[q]) + dtd * inc;
+                        const double x = ((double)e1[q] + (comp ? (double)e1l[q] : 0.0)) + dtd * inc;
                         const R hi = R(x);
                         st_at(qo + q * fs, coff, hi);
                         if (wlo) st_at(qol + q * fs, coff, R(x - (double)hi));
@@ -605,7 +589,8 @@ __device__ __forceinline__ void rhs_tile(const Params &P, const StagePtrs<R> &A,
                     R *const aol = comp ? arr(10, poff) : ao;
 #pragma unroll
                     for (int q = 0; q < 5; q++) {
-                        const double x = ((double)e2[q] + (double)e2l[q]) + dtd * (s_w[3] * (double)rhs[q]);
+                        const double x =
+                            ((double)e3[q] + (comp ? (double)e3l[q] : 0.0)) + dtd * (s_w[3] * (double)rhs[q]);
                         const R hi = R(x);
                         st_at(ao + q * fs, coff, hi);
                         if (comp) st_at(aol + q * fs, coff, R(x - (double)hi));
@@ -618,7 +603,7 @@ __device__ __forceinline__ void rhs_tile(const Params &P, const StagePtrs<R> &A,
                     for (int q = 0; q < 5; q++) {
                         R inc = R(0);
                         if (fl & F_RA) inc += R(s_w[0]) * ea[q];
-                        if (fl & F_RB) inc += R(s_w[1]) * eb[q];
+                        if (fl & F_RB) inc += R(s_w[1]) * e3[q];
                         inc += R(s_w[2]) * rhs[q];
                         st_at(qo + q * fs, coff, e1[q] + dt * inc);
                     }
@@ -631,7 +616,7 @@ __device__ __forceinline__ void rhs_tile(const Params &P, const StagePtrs<R> &A,
                 if (fl & F_ACC) {
                     R *const ao = arr(6, poff);
 #pragma unroll
-                    for (int q = 0; q < 5; q++) st_at(ao + q * fs, coff, e2[q] + dt * (R(s_w[3]) * rhs[q]));
+                    for (int q = 0; q < 5; q++) st_at(ao + q * fs, coff, e3[q] + dt * (R(s_w[3]) * rhs[q]));
                 }
             }
         }
@@ -718,40 +703,41 @@ __device__ __forceinline__ void dil_tile(const Params &P, const StagePtrs<R> &A,
         v2[0] = ld_at(b, coff);
         v2[1] = ld_at(b + 3 * fs, coff);
     };
+    // unconditional loads (see rhs_tile): lanes past the edge and lanes
+    // without a halo point load valid points they do not use
     R ww[W];
 #pragma unroll
-    for (int a = 0; a < W; a++) ww[a] = R(0);
-    if (act) {
-#pragma unroll
-        for (int a = 0; a < W - 1; a++) {
-            R t[2];
-            ldw((long)wrapi(z0 - H + a, P.mz) * mxy, t);
-            ww[a] = t[1] / rho(t[0]);
-        }
+    for (int a = 0; a < W - 1; a++) {
+        R t[2];
+        ldw((long)wrapi(z0 - H + a, P.mz) * mxy, t);
+        ww[a] = t[1] / rho(t[0]);
     }
-    R zr[2] = {R(1) - r0, R(0)}, cr[3], hr[3] = {R(1) - r0, R(0), R(0)};
-    if (act) ldw((long)wrapi(z0 + H, P.mz) * mxy, zr);
+    ww[W - 1] = R(0);
+    R zr[2], cr[3], hr[3];
+    ldw((long)wrapi(z0 + H, P.mz) * mxy, zr);
     ld3((long)z0 * mxy, coff, cr);
-    if (hact) ld3((long)z0 * mxy, hoff, hr);
+    ld3((long)z0 * mxy, hoff, hr);
 
     for (int k = z0; k < z1; ++k) {
         const int b = (k - z0) & 1;
         const long poff = (long)k * mxy;
-        const bool more = k + 1 < z1;
+        const long pnext = (long)wrapi(k + 1, P.mz) * mxy;
         ww[W - 1] = zr[1] / rho(zr[0]);
-        if (act && more) ldw((long)wrapi(k + 1 + H, P.mz) * mxy, zr);
+        ldw((long)wrapi(k + 1 + H, P.mz) * mxy, zr);
         {
             const R inv = R(1) / rho(cr[0]);
             su[b][cc] = csg * (cr[1] * inv);
             sv[b][cc] = csg * (cr[2] * inv);
         }
-        if (more) ld3(poff + mxy, coff, cr);
-        if (hact) {
+        ld3(pnext, coff, cr);
+        {
             const R inv = R(1) / rho(hr[0]);
-            su[b][hslot] = hsg * (hr[1] * inv);
-            sv[b][hslot] = hsg * (hr[2] * inv);
-            if (more) ld3(poff + mxy, hoff, hr);
+            if (hact) {
+                su[b][hslot] = hsg * (hr[1] * inv);
+                sv[b][hslot] = hsg * (hr[2] * inv);
+            }
         }
+        ld3(pnext, hoff, hr);
         __syncthreads();
         if (act) {
             R a1 = R(0), a2 = R(0), a3 = R(0);
