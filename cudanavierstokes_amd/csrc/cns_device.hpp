@@ -189,6 +189,12 @@ __device__ __forceinline__ void wall_ghost(const Params &P, R tw, R &r, R &u, R 
     r = p / (R(P.Rgas) * t);
 }
 
+// 1/x: correctly rounded division for FP64 storage (parity with the FP64
+// reference); the hardware reciprocal (1 ulp) for FP32 storage, whose
+// tolerance is 1e-5.
+__device__ __forceinline__ double recip(double x) { return 1.0 / x; }
+__device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
+
 // Equation of state on offset storage (FP32): from rho' = rho - rho0 and
 // E' = rhoE - E0,
 //   p' = (gam-1)(E' - rho |u|^2/2),  T' = (p' - rho' R T0)/(rho R),
@@ -199,7 +205,7 @@ template <typename R, bool UNITV>
 __device__ __forceinline__ void eos_off(const Params &P, R rp, R ru, R rv, R rw, R ep, R &u, R &v, R &w, R &tp,
                                         R &hp, R &mu) {
     const R r = R(P.rho0) + rp;
-    const R inv = R(1) / r;
+    const R inv = recip(r);
     u = ru * inv;
     v = rv * inv;
     w = rw * inv;

@@ -710,7 +710,7 @@ __device__ __forceinline__ void dil_tile(const Params &P, const StagePtrs<R> &A,
     for (int a = 0; a < W - 1; a++) {
         R t[2];
         ldw((long)wrapi(z0 - H + a, P.mz) * mxy, t);
-        ww[a] = t[1] / rho(t[0]);
+        ww[a] = t[1] * recip(rho(t[0]));
     }
     ww[W - 1] = R(0);
     R zr[2], cr[3], hr[3];
@@ -722,16 +722,16 @@ __device__ __forceinline__ void dil_tile(const Params &P, const StagePtrs<R> &A,
         const int b = (k - z0) & 1;
         const long poff = (long)k * mxy;
         const long pnext = (long)wrapi(k + 1, P.mz) * mxy;
-        ww[W - 1] = zr[1] / rho(zr[0]);
+        ww[W - 1] = zr[1] * recip(rho(zr[0]));
         ldw((long)wrapi(k + 1 + H, P.mz) * mxy, zr);
         {
-            const R inv = R(1) / rho(cr[0]);
+            const R inv = recip(rho(cr[0]));
             su[b][cc] = csg * (cr[1] * inv);
             sv[b][cc] = csg * (cr[2] * inv);
         }
         ld3(pnext, coff, cr);
         {
-            const R inv = R(1) / rho(hr[0]);
+            const R inv = recip(rho(hr[0]));
             if (hact) {
                 su[b][hslot] = hsg * (hr[1] * inv);
                 sv[b][hslot] = hsg * (hr[2] * inv);

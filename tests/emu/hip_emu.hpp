@@ -68,6 +68,9 @@ inline void __builtin_amdgcn_raw_buffer_store_b32(unsigned v, __amdgpu_buffer_rs
     std::memcpy(r.base + (size_t)(unsigned)vo + (size_t)(unsigned)so, &v, 4);
 }
 
+// hardware reciprocal: IEEE division on the CPU
+inline float __builtin_amdgcn_rcpf(float x) { return 1.0f / x; }
+
 // scheduling hints and wave-uniform reads have no effect on the CPU
 #define __builtin_amdgcn_sched_barrier(m) ((void)(m))
 inline int __builtin_amdgcn_readfirstlane(int v) { return v; }
