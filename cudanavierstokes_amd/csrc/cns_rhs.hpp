@@ -301,6 +301,11 @@ __device__ __forceinline__ void block_max_atomic(double v, unsigned long long *o
     }
 }
 
+template <typename R>
+__device__ __forceinline__ void phase_fence() {
+    if (sizeof(R) == 8) __builtin_amdgcn_sched_barrier(0);
+}
+
 // Bits of the stage-update flags.
 enum { F_Q = 1, F_RA = 2, F_RB = 4, F_ACC = 8, F_R = 16, F_SPONGE = 32, F_VISC = 64, F_COMP = 128, F_QLO = 256 };
 
@@ -511,27 +516,29 @@ __device__ __forceinline__ void rhs_tile(const Params &P, const StagePtrs<R> &A,
                 return (UNITV && q == QM) ? (Off<R>::on ? T0 + win[QT][H + a] : win[QT][H + a]) * invRe
                                           : win[q][H + a];
             };
-            // scheduling barriers between the phases keep the compiler from
-            // hoisting every LDS read of the point to the top (register pressure)
+            // FP64: scheduling barriers between the phases keep the compiler
+            // from hoisting every LDS read of the point to the top (register
+            // pressure at one wave per SIMD); the FP32 kernel schedules
+            // better without them
             R d1x[3], d2x[3], d1y[3], d2y[3], d1z[3], d2z[3];
             vel_stencils<R, SV, 0, NONUNI>(P, FX, vs, xpi, d1x, d2x);
-            __builtin_amdgcn_sched_barrier(0);
+            phase_fence<R>();
             vel_stencils<R, SV, 1, false>(P, FY, vs, xpi, d1y, d2y);
-            __builtin_amdgcn_sched_barrier(0);
+            phase_fence<R>();
             vel_stencils<R, SV, 2, false>(P, FZ, vs, xpi, d1z, d2z);
-            __builtin_amdgcn_sched_barrier(0);
+            phase_fence<R>();
             const R g[9] = {d1x[0], d1x[1], d1x[2], d1y[0], d1y[1], d1y[2], d1z[0], d1z[1], d1z[2]};
             // x part (=), y part (+=), z part (+=) with the forcing, as
             // src/cuda_rhs.cu:123-127,255-259,389-393 accumulate them
             R rhs[5], part[5];
             dir_terms<R, S, SV, 0, NONUNI, UNITV>(P, FX, vs, xpi, g, d2x, rhs);
-            __builtin_amdgcn_sched_barrier(0);
+            phase_fence<R>();
             dir_terms<R, S, SV, 1, false, UNITV>(P, FY, vs, xpi, g, d2y, part);
 #pragma unroll
             for (int q = 0; q < 5; q++) rhs[q] = rhs[q] + part[q];
-            __builtin_amdgcn_sched_barrier(0);
+            phase_fence<R>();
             dir_terms<R, S, SV, 2, false, UNITV>(P, FZ, vs, xpi, g, d2z, part);
-            __builtin_amdgcn_sched_barrier(0);
+            phase_fence<R>();
             const R wc = win[QW][H];
             rhs[0] = rhs[0] + part[0];
             rhs[1] = rhs[1] + part[1];
