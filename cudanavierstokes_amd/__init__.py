@@ -5,8 +5,8 @@ C ABI of include/cns.h); this package is its Python host binding.
 """
 from ._lib import CnsConfig, CnsError, load  # noqa: F401
 from .solver import (Batch, Config, Solver, channel, clearSolver, copyField,  # noqa: F401
-                     initSolver, runSimulation, setDevice, setGPUParameters,
-                     taylor_green)
+                     initField, initSolver, runSimulation, setDevice, setGPUParameters,
+                     taylor_green, writeField)
 
 __all__ = ["Batch", "Config", "Solver", "CnsError", "taylor_green", "channel", "setDevice",
-           "setGPUParameters", "initSolver", "copyField", "runSimulation", "clearSolver", "load"]
+           "setGPUParameters", "initSolver", "copyField", "runSimulation", "clearSolver", "writeField", "initField", "load"]

@@ -32,6 +32,14 @@ class CnsConfig(C.Structure):
     ]
 
 
+class CnsControl(C.Structure):
+    """Mirror of struct cns_control."""
+    _fields_ = [
+        ("time", C.c_double), ("dt", C.c_double), ("dpdz", C.c_double), ("visc_max", C.c_double),
+        ("steps", C.c_int64), ("visc_from_state", C.c_int), ("reserved", C.c_int),
+    ]
+
+
 # Python Config names (the reference's macro names) -> C struct fields
 C_NAMES = {"mx": "nx", "my": "ny", "mz": "nz", "forcing": "forcing_z", "Lx": "len_x", "Ly": "len_y",
            "Lz": "len_z", "Re": "reynolds", "Pr": "prandtl", "Ma": "mach", "gam": "gamma",
@@ -43,6 +51,7 @@ EXPORTS = [
     "cns_fill_taylor_green", "cns_fill_channel", "cns_init_taylor_green", "cns_init_channel", "cns_set_state", "cns_get_state", "cns_rhs",
     "cns_step", "cns_set_sponge", "cns_get_scalars", "cns_sync", "cns_profile_enable",
     "cns_profile_read", "cns_time_steps", "cns_device_bytes",
+    "cns_write_fields", "cns_read_fields", "cns_write_grid", "cns_get_control", "cns_set_control",
     "cns_batch_create", "cns_batch_tune", "cns_batch_step", "cns_batch_launch", "cns_batch_wait", "cns_batch_time_steps", "cns_batch_group_info", "cns_batch_set_shape",
     "cns_batch_groups", "cns_batch_destroy", "cns_batch_last_error",
 ]
@@ -86,6 +95,11 @@ def load(path=None):
     L.cns_profile_read.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
     L.cns_time_steps.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
     L.cns_device_bytes.argtypes = [vp, C.POINTER(C.c_uint64)]
+    L.cns_write_fields.argtypes = [vp, C.c_char_p, C.c_int]
+    L.cns_read_fields.argtypes = [vp, C.c_char_p, C.c_int]
+    L.cns_write_grid.argtypes = [vp, C.c_char_p]
+    L.cns_get_control.argtypes = [vp, C.POINTER(CnsControl)]
+    L.cns_set_control.argtypes = [vp, C.POINTER(CnsControl)]
     L.cns_batch_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(vp)]
     L.cns_batch_tune.argtypes = [vp]
     L.cns_batch_step.argtypes = [vp, C.c_int]

@@ -10,9 +10,12 @@
 //   cns_step          runSimulation / runSimulationLowStorage (src/cuda_main.cu:44-186)
 //   dt, forcing       calcTimeStepPressGrad (src/cuda_main.cu:249-265), kept on the device
 //   cns_destroy       clearSolver (src/cuda_utils.cu:521)
+//   field files       writeField / initField (src/init.cpp:13-30), saveFileMPI /
+//                     readFileMPI (src/comm.cpp:205-279), grid files (src/init.cpp:78-88)
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cerrno>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -1171,6 +1174,135 @@ extern "C" int cns_profile_read(cns_solver *s, double *total_ms, int *launches) 
 extern "C" int cns_device_bytes(const cns_solver *s, uint64_t *bytes) {
     if (!s || !bytes) return fail(nullptr, CNS_ERR_ARG, "cns_device_bytes: null argument");
     *bytes = s->dev_bytes;
+    return CNS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Field files (src/comm.cpp:205-279, src/init.cpp:13-30,78-88) and control
+// ---------------------------------------------------------------------------
+
+static const char FIELD_NAMES[5] = {'r', 'u', 'v', 'w', 'e'};
+
+static std::string field_path(const char *dir, char name, int timestep) {
+    char f[32];
+    std::snprintf(f, sizeof(f), "%c.%07d.bin", name, timestep);
+    return std::string(dir) + "/" + f;
+}
+
+// writes count doubles to path; false on any failure (the message in err)
+static bool write_f64(const std::string &path, const double *a, size_t count, std::string &err) {
+    std::FILE *f = std::fopen(path.c_str(), "wb");
+    if (!f) {
+        err = "cannot open " + path + " for writing: " + std::strerror(errno);
+        return false;
+    }
+    const size_t got = std::fwrite(a, sizeof(double), count, f);
+    const bool closed = std::fclose(f) == 0;
+    if (got != count || !closed) {
+        err = "short write to " + path;
+        return false;
+    }
+    return true;
+}
+
+// reads exactly count doubles from path, which must hold no more and no less
+static bool read_f64(const std::string &path, double *a, size_t count, std::string &err) {
+    std::FILE *f = std::fopen(path.c_str(), "rb");
+    if (!f) {
+        err = "cannot open " + path + ": " + std::strerror(errno);
+        return false;
+    }
+    bool ok = std::fseek(f, 0, SEEK_END) == 0;
+    const long size = ok ? std::ftell(f) : -1;
+    ok = ok && size >= 0 && (size_t)size == count * sizeof(double) && std::fseek(f, 0, SEEK_SET) == 0;
+    if (!ok) {
+        std::fclose(f);
+        err = path + " holds " + std::to_string(size) + " bytes, expected " +
+              std::to_string(count * sizeof(double)) + " (float64 of the whole grid)";
+        return false;
+    }
+    const size_t got = std::fread(a, sizeof(double), count, f);
+    std::fclose(f);
+    if (got != count) {
+        err = "short read from " + path;
+        return false;
+    }
+    return true;
+}
+
+extern "C" int cns_write_fields(cns_solver *s, const char *dir, int timestep) {
+    if (!s || !dir) return fail(s, CNS_ERR_ARG, "cns_write_fields: null argument");
+    if (timestep < 0 || timestep > 9999999) return fail(s, CNS_ERR_ARG, "cns_write_fields: timestep outside 0..9999999");
+    const size_t n = (size_t)s->n;
+    std::vector<double> q(5 * n);
+    int rc = cns_get_state(s, q.data(), q.data() + n, q.data() + 2 * n, q.data() + 3 * n, q.data() + 4 * n);
+    if (rc) return rc;
+    std::string err;
+    for (int k = 0; k < 5; k++)
+        if (!write_f64(field_path(dir, FIELD_NAMES[k], timestep), q.data() + k * n, n, err))
+            return fail(s, CNS_ERR_IO, "cns_write_fields: " + err);
+    return CNS_OK;
+}
+
+extern "C" int cns_read_fields(cns_solver *s, const char *dir, int timestep) {
+    if (!s || !dir) return fail(s, CNS_ERR_ARG, "cns_read_fields: null argument");
+    if (timestep < 0 || timestep > 9999999) return fail(s, CNS_ERR_ARG, "cns_read_fields: timestep outside 0..9999999");
+    if (s->in_batch) return fail(s, CNS_ERR_STATE, "cns_read_fields: the solver is advanced by a batch");
+    const size_t n = (size_t)s->n;
+    std::vector<double> q(5 * n);
+    std::string err;
+    for (int k = 0; k < 5; k++)
+        if (!read_f64(field_path(dir, FIELD_NAMES[k], timestep), q.data() + k * n, n, err))
+            return fail(s, CNS_ERR_IO, "cns_read_fields: " + err);
+    return cns_set_state(s, q.data(), q.data() + n, q.data() + 2 * n, q.data() + 3 * n, q.data() + 4 * n);
+}
+
+extern "C" int cns_write_grid(const cns_solver *s, const char *dir) {
+    if (!s || !dir) return fail(nullptr, CNS_ERR_ARG, "cns_write_grid: null argument");
+    std::string err;
+    const std::string d(dir);
+    if (!write_f64(d + "/x.bin", s->x.data(), s->x.size(), err) ||
+        !write_f64(d + "/y.bin", s->y.data(), s->y.size(), err) ||
+        !write_f64(d + "/z.bin", s->z.data(), s->z.size(), err))
+        return fail(const_cast<cns_solver *>(s), CNS_ERR_IO, "cns_write_grid: " + err);
+    return CNS_OK;
+}
+
+extern "C" int cns_get_control(cns_solver *s, cns_control *c) {
+    if (!s || !c) return fail(s, CNS_ERR_ARG, "cns_get_control: null argument");
+    if (s->in_batch) return fail(s, CNS_ERR_STATE, "cns_get_control: the solver is advanced by a batch");
+    HIPCHK(s, hipSetDevice(s->dev));
+    Scalars h{};
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    HIPCHK(s, hipMemcpy(&h, s->d_sc, sizeof(h), hipMemcpyDeviceToHost));
+    std::memset(c, 0, sizeof(*c));
+    c->time = h.time;
+    c->dt = h.dt;
+    c->dpdz = h.dpdz;
+    // the maximum is kept as the bits of a non-negative double
+    std::memcpy(&c->visc_max, &h.visc_max, sizeof(double));
+    c->steps = s->bsteps;
+    c->visc_from_state = s->visc_from_state ? 1 : 0;
+    return CNS_OK;
+}
+
+extern "C" int cns_set_control(cns_solver *s, const cns_control *c) {
+    if (!s || !c) return fail(s, CNS_ERR_ARG, "cns_set_control: null argument");
+    if (s->in_batch) return fail(s, CNS_ERR_STATE, "cns_set_control: the solver is advanced by a batch");
+    if (!s->have_state) return fail(s, CNS_ERR_STATE, "cns_set_control: no state set");
+    if (!(c->dt >= 0.0) || !(c->visc_max >= 0.0) || c->steps < 0 || !std::isfinite(c->time))
+        return fail(s, CNS_ERR_ARG, "cns_set_control: dt, visc_max and steps must be non-negative, time finite");
+    HIPCHK(s, hipSetDevice(s->dev));
+    Scalars h{};
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    HIPCHK(s, hipMemcpy(&h, s->d_sc, sizeof(h), hipMemcpyDeviceToHost));
+    h.time = c->time;
+    h.dt = c->dt;
+    h.dpdz = c->dpdz;
+    std::memcpy(&h.visc_max, &c->visc_max, sizeof(double));
+    HIPCHK(s, hipMemcpy(s->d_sc, &h, sizeof(h), hipMemcpyHostToDevice));
+    s->bsteps = c->steps;
+    s->visc_from_state = c->visc_from_state != 0;
     return CNS_OK;
 }
 

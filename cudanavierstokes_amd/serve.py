@@ -11,16 +11,26 @@ admitted, and the batch is rebuilt for the new membership.  Each request's
 result equals its lone cns_step result to rounding (the batch keeps every
 member's step count, so the CFL checks fall on the same steps).
 
+A request may start from a checkpoint instead of its initial condition
+(restart=dict(dir=..., timestep=...): field files plus the time-advance
+control, see fields.save_checkpoint), and with checkpoint_dir set every
+finished request leaves one in <checkpoint_dir>/<id>/, numbered by the steps
+taken (plus the restart timestep), so a long run can be served in parts.  The
+checkpoint carries the CFL check phase, so a resumed request continues as the
+uninterrupted one would, to rounding.
+
 Run as a module for the seeded serving workload:
     python -m cudanavierstokes_amd.serve --requests 1000 --rate 20
 """
 import argparse
 import collections
 import json
+import os
 import time
 
 import numpy as np
 
+from .fields import load_checkpoint, save_checkpoint
 from .solver import Batch, Solver
 from .workloads import init_on_device, points, serving
 
@@ -40,8 +50,10 @@ def footprint(config):
 
 
 class Server:
-    def __init__(self, budget_bytes=200e9, tick_steps=10, max_members=512, tune=False, keep_results=False):
+    def __init__(self, budget_bytes=200e9, tick_steps=10, max_members=512, tune=False, keep_results=False,
+                 checkpoint_dir=None):
         self.budget = float(budget_bytes)
+        self.ckpt = checkpoint_dir
         self.tick = int(tick_steps)
         self.max_members = int(max_members)
         self.tune = tune
@@ -80,7 +92,10 @@ class Server:
                 waiting.popleft()
                 ta = time.perf_counter()
                 s = Solver(r["config"])
-                init_on_device(s, r["kind"], r.get("seed", 1))
+                if "restart" in r:
+                    load_checkpoint(s, r["restart"]["dir"], r["restart"]["timestep"])
+                else:
+                    init_on_device(s, r["kind"], r.get("seed", 1))
                 cost["admit_s"] += time.perf_counter() - ta
                 joining[r["id"]] = dict(req=r, solver=s, left=int(r["steps"]), admitted=clock(), bytes=need)
                 used[0] += need
@@ -132,6 +147,11 @@ class Server:
                                points=points(r["config"]))
                     if self.keep:
                         rec["state"] = state
+                    if self.ckpt is not None:
+                        ts = int(r["steps"]) + (int(r["restart"]["timestep"]) if "restart" in r else 0)
+                        d = os.path.join(self.ckpt, str(r["id"]))
+                        save_checkpoint(a["solver"], d, ts)
+                        rec["checkpoint"] = dict(dir=d, timestep=ts)
                     done.append(rec)
                     a["solver"].close()
                     used[0] -= a["bytes"]

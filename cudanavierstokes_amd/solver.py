@@ -7,18 +7,21 @@ Reference entry points (src/main.h:23-42) and their counterparts here:
   initSolver(Communicator)       -> initSolver()             (device buffers)
   copyField(direction, ...)      -> copyField(0, fields) host->device, copyField(1) device->host
   runSimulation(kin, enst, time) -> runSimulation(nsteps)    (returns kin, enst, time)
+  writeField(timestep)           -> writeField(timestep, directory="fields")
+  initField(timestep)            -> initField(timestep, directory="fields")
   clearSolver(Communicator)      -> clearSolver()
 
 Solver is the handle-based form of the same interface; several instances can
 coexist, unlike the reference's compile-time globals.
 """
 import ctypes as C
+import os
 from dataclasses import dataclass, fields
 
 import numpy as np
 
 from . import _lib
-from ._lib import CnsConfig, CnsError, check, check_batch
+from ._lib import CnsConfig, CnsControl, CnsError, check, check_batch
 
 SCHEMES = {"rk3": 0, "smr_rk3": 0, "lowstorage": 0, "kutta_rk3": 1, "rk4": 2}
 
@@ -207,6 +210,35 @@ class Solver:
         self._chk(self.L.cns_device_bytes(self.h, C.byref(b)), "cns_device_bytes")
         return b.value
 
+    def write_fields(self, directory, timestep):
+        """<directory>/{r,u,v,w,e}.<timestep:07d>.bin, float64 [mz][my][mx]
+        (writeField, src/init.cpp:24-30)"""
+        self._chk(self.L.cns_write_fields(self.h, os.fsencode(directory), int(timestep)), "cns_write_fields")
+
+    def read_fields(self, directory, timestep):
+        """state from the five field files of a timestep (initField,
+        src/init.cpp:13-22); time restarts at zero"""
+        self._chk(self.L.cns_read_fields(self.h, os.fsencode(directory), int(timestep)), "cns_read_fields")
+
+    def write_grid(self, directory):
+        """<directory>/{x,y,z}.bin node coordinates (src/init.cpp:78-88)"""
+        self._chk(self.L.cns_write_grid(self.h, os.fsencode(directory)), "cns_write_grid")
+
+    def control(self):
+        """time, dt, dpdz, the viscous maximum pending for the next CFL
+        check, the batched-step CFL phase and whether the next check takes
+        the viscous limit from the state"""
+        c = CnsControl()
+        self._chk(self.L.cns_get_control(self.h, C.byref(c)), "cns_get_control")
+        return {f: getattr(c, f) for f, _ in CnsControl._fields_ if f != "reserved"}
+
+    def set_control(self, ctl):
+        c = CnsControl()
+        for f, _ in CnsControl._fields_:
+            if f != "reserved":
+                setattr(c, f, ctl[f])
+        self._chk(self.L.cns_set_control(self.h, C.byref(c)), "cns_set_control")
+
 
 class Batch:
     """Many solvers advanced together (cns_batch_*): one launch per RK stage
@@ -320,6 +352,16 @@ def copyField(direction, fields_=None):
 def runSimulation(nsteps):
     """nsteps steps of the configured scheme; returns (kin, enst, time)."""
     return _solver().step(nsteps)
+
+
+def writeField(timestep, directory="fields"):
+    """the five field files of the current state (src/init.cpp:24-30)"""
+    _solver().write_fields(directory, timestep)
+
+
+def initField(timestep, directory="fields"):
+    """state from the field files of a timestep (src/init.cpp:13-22)"""
+    _solver().read_fields(directory, timestep)
 
 
 def clearSolver():

@@ -34,6 +34,7 @@ extern "C" {
 #define CNS_ERR_STATE (-3)
 #define CNS_ERR_UNSUPPORTED (-4)
 #define CNS_ERR_NOMEM (-5)
+#define CNS_ERR_IO (-6)
 
 /* time-advance schemes */
 #define CNS_RK3_LOWSTORAGE 0 /* Spalart-Moser-Rogers RK3, src/cuda_main.cu:109-186 */
@@ -125,6 +126,39 @@ int cns_set_sponge(cns_solver *s, const double *sigma_x, const double *sigma_z, 
 
 /* current dt, dp/dz and simulation time */
 int cns_get_scalars(cns_solver *s, double *dt, double *dpdz, double *time);
+
+/* Field files of writeField / initField (src/init.cpp:13-30), written and
+ * read like saveFileMPI / readFileMPI (src/comm.cpp:205-279):
+ * <dir>/<c>.<timestep, %07d>.bin for c = r, u, v, w, e, each the whole grid
+ * as native-endian float64 in [nz][ny][nx] order (x unit-stride), variables
+ * (rho, u, v, w, rhoE).  dir is the reference's "fields" directory and must
+ * exist.  cns_read_fields sets the state as cns_set_state does (time restarts
+ * at zero, as in the reference's restart); a missing file or one whose size
+ * is not nx*ny*nz*8 bytes is CNS_ERR_IO and leaves the solver unchanged. */
+int cns_write_fields(cns_solver *s, const char *dir, int timestep);
+int cns_read_fields(cns_solver *s, const char *dir, int timestep);
+
+/* grid files of initGrid (src/init.cpp:78-88): <dir>/x.bin, y.bin, z.bin,
+ * the node coordinates as float64 (nx, ny, nz values) */
+int cns_write_grid(const cns_solver *s, const char *dir);
+
+/* Time-advance control besides the state: with it a run resumed from field
+ * files continues as the uninterrupted run would (to the rounding of the
+ * u = rho u / rho round trip of the file format).  No reference counterpart:
+ * the reference's restart starts time at zero and dp/dz at its initial value
+ * (src/cuda_main.cu:286-291, src/cuda_utils.cu:68-70).  Set it after the
+ * state, which resets it. */
+typedef struct cns_control {
+    double time;          /* simulation time */
+    double dt;            /* time step set by the last CFL check */
+    double dpdz;          /* forcing */
+    double visc_max;      /* viscous maximum gathered for the next CFL check */
+    int64_t steps;        /* batched steps since the last cns_step / state set (CFL check phase) */
+    int visc_from_state;  /* 1: the next CFL check takes the viscous limit from the state */
+    int reserved;
+} cns_control;
+int cns_get_control(cns_solver *s, cns_control *c);
+int cns_set_control(cns_solver *s, const cns_control *c);
 
 /* Wait for all work queued on the solver's stream. */
 int cns_sync(cns_solver *s);
