@@ -10,6 +10,7 @@
 //   cns_step          runSimulation / runSimulationLowStorage (src/cuda_main.cu:44-186)
 //   dt, forcing       calcTimeStepPressGrad (src/cuda_main.cu:249-265), kept on the device
 //   cns_destroy       clearSolver (src/cuda_utils.cu:521)
+//   diagnostics       calcAvgChan, printRes (src/init.cpp:150-256) on the device
 //   field files       writeField / initField (src/init.cpp:13-30), saveFileMPI /
 //                     readFileMPI (src/comm.cpp:205-279), grid files (src/init.cpp:78-88)
 #include <hip/hip_runtime.h>
@@ -28,6 +29,7 @@
 #include "../../include/cns.h"
 #include "cns_aux.hpp"
 #include "cns_batch.hpp"
+#include "cns_diag.hpp"
 #include "cns_launch.hpp"
 
 using namespace cns;
@@ -147,6 +149,7 @@ struct cns_solver {
     void *rbuf[2] = {nullptr};  // right-hand-side sets, 5*n each
     int cur = 0;
     double *d_par1 = nullptr, *d_par2 = nullptr, *d_time = nullptr;
+    double *d_prof = nullptr;  // 10*mx wall-normal profiles (allocated on first use)
     int par_cap = 0;
     bool visc_from_state = true;
     bool have_state = false;
@@ -445,6 +448,7 @@ extern "C" int cns_destroy(cns_solver *s) {
     fr(s->d_par1);
     fr(s->d_par2);
     fr(s->d_time);
+    fr(s->d_prof);
     for (auto &e : s->ev) (void)hipEventDestroy(e);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
@@ -1175,6 +1179,134 @@ extern "C" int cns_device_bytes(const cns_solver *s, uint64_t *bytes) {
     if (!s || !bytes) return fail(nullptr, CNS_ERR_ARG, "cns_device_bytes: null argument");
     *bytes = s->dev_bytes;
     return CNS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Diagnostics of the current state (cns_diag.hpp)
+// ---------------------------------------------------------------------------
+
+// stored state set and its low part (null unless compensated FP32)
+template <typename R>
+static void state_arrays(const cns_solver *s, const R *&q, const R *&lo) {
+    q = as<R>(s->sets[s->cur]);
+    lo = (s->fp32 && s->comp) ? as<R>(s->lo[s->cur]) : nullptr;
+}
+
+static int diag_ready(cns_solver *s, const char *what) {
+    if (s->in_batch) return fail(s, CNS_ERR_STATE, std::string(what) + ": the solver is advanced by a batch");
+    if (!s->have_state) return fail(s, CNS_ERR_STATE, std::string(what) + ": no state set");
+    HIPCHK(s, hipSetDevice(s->dev));
+    return CNS_OK;
+}
+
+template <typename R>
+static int xprof_t(cns_solver *s, double *prof) {
+    const int mx = s->cfg.nx, my = s->cfg.ny, mz = s->cfg.nz;
+    const long rows = (long)my * mz;
+    if (!s->d_prof) {
+        int rc = balloc(s, (void **)&s->d_prof, 10 * (size_t)mx * sizeof(double));
+        if (rc) return rc;
+    }
+    const R *q, *lo;
+    state_arrays<R>(s, q, lo);
+    // chunk partials in the first rhs set (free between steps):
+    // nch * 5 * mx doubles within its 5 * n * esz bytes
+    double *part = static_cast<double *>(s->rbuf[0]);
+    const int nch = (int)std::max(1L, std::min(256L, rows * (long)s->esz / 8));
+    const dim3 grid((mx + 255) / 256, nch), fin((mx + 255) / 256);
+    for (int pass = 0; pass < 2; pass++) {
+        hipLaunchKernelGGL(k_xprof_sums<R>, grid, dim3(256), 0, s->stream, mx, rows, (long)s->n, q, lo, s->rho0, s->E0,
+                           pass, (const double *)s->d_prof, part, nch);
+        LAUNCHED(s, "k_xprof_sums");
+        hipLaunchKernelGGL(k_xprof_final, fin, dim3(256), 0, s->stream, mx, my, mz, (const double *)part, nch, pass,
+                           s->d_prof);
+        LAUNCHED(s, "k_xprof_final");
+    }
+    std::copy(s->x.begin(), s->x.end(), prof);
+    HIPCHK(s, hipMemcpyAsync(prof + mx, s->d_prof, 10 * (size_t)mx * sizeof(double), hipMemcpyDeviceToHost,
+                             s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    return CNS_OK;
+}
+
+extern "C" int cns_channel_profiles(cns_solver *s, double *prof) {
+    if (!s || !prof) return fail(s, CNS_ERR_ARG, "cns_channel_profiles: null argument");
+    int rc = diag_ready(s, "cns_channel_profiles");
+    if (rc) return rc;
+    return s->fp32 ? xprof_t<float>(s, prof) : xprof_t<double>(s, prof);
+}
+
+template <typename R, int S>
+static void launch_ret(cns_solver *s, const R *q, const R *lo) {
+    const double muw = std::pow(1.0, s->cfg.visc_exp) / s->cfg.reynolds;
+    hipLaunchKernelGGL((k_ret<R, S>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s->stream, s->cfg.nx,
+                       (long)s->cfg.ny * s->cfg.nz, (long)s->n, q, lo, s->rho0, s->E0, s->dx, s->xp[0], muw, s->cF[0],
+                       s->cF[1], s->cF[2], s->cF[3], s->d_part);
+}
+
+template <typename R>
+static int ret_t(cns_solver *s, double *re_tau) {
+    const R *q, *lo;
+    state_arrays<R>(s, q, lo);
+    switch (s->S) {
+        case 1: launch_ret<R, 1>(s, q, lo); break;
+        case 2: launch_ret<R, 2>(s, q, lo); break;
+        case 3: launch_ret<R, 3>(s, q, lo); break;
+        default: launch_ret<R, 4>(s, q, lo); break;
+    }
+    LAUNCHED(s, "k_ret");
+    std::vector<double> part(RED_BLOCKS);
+    HIPCHK(s, hipMemcpyAsync(part.data(), s->d_part, part.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    double a = 0.0;
+    for (double v : part) a += v;
+    *re_tau = a / s->cfg.ny / s->cfg.nz;
+    return CNS_OK;
+}
+
+extern "C" int cns_friction_reynolds(cns_solver *s, double *re_tau) {
+    if (!s || !re_tau) return fail(s, CNS_ERR_ARG, "cns_friction_reynolds: null argument");
+    int rc = diag_ready(s, "cns_friction_reynolds");
+    if (rc) return rc;
+    return s->fp32 ? ret_t<float>(s, re_tau) : ret_t<double>(s, re_tau);
+}
+
+template <typename R, int SV>
+static void launch_stats(cns_solver *s, const R *q, const R *lo) {
+    hipLaunchKernelGGL((k_flow_stats<R, SV>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s->stream, s->base,
+                       (long)s->n, q, lo, s->cfg.periodic_x ? 0 : 1, s->cfg.nonuniform_x ? 1 : 0, s->d_part);
+}
+
+template <typename R>
+static int stats_t(cns_solver *s, double *out) {
+    const R *q, *lo;
+    state_arrays<R>(s, q, lo);
+    switch (s->SV) {
+        case 1: launch_stats<R, 1>(s, q, lo); break;
+        case 2: launch_stats<R, 2>(s, q, lo); break;
+        case 3: launch_stats<R, 3>(s, q, lo); break;
+        default: launch_stats<R, 4>(s, q, lo); break;
+    }
+    LAUNCHED(s, "k_flow_stats");
+    std::vector<double> part(4 * RED_BLOCKS);
+    HIPCHK(s, hipMemcpyAsync(part.data(), s->d_part, part.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    double sum[4] = {0, 0, 0, 0};
+    for (int a = 0; a < 4; a++)
+        for (int b = 0; b < RED_BLOCKS; b++) sum[a] += part[a * RED_BLOCKS + b];
+    const double vol = s->cfg.len_x * s->cfg.len_y * s->cfg.len_z;
+    out[0] = sum[1] / sum[0];
+    out[1] = sum[2] / sum[0];
+    out[2] = sum[3] / vol;
+    out[3] = sum[0] / vol;
+    return CNS_OK;
+}
+
+extern "C" int cns_flow_stats(cns_solver *s, double *out) {
+    if (!s || !out) return fail(s, CNS_ERR_ARG, "cns_flow_stats: null argument");
+    int rc = diag_ready(s, "cns_flow_stats");
+    if (rc) return rc;
+    return s->fp32 ? stats_t<float>(s, out) : stats_t<double>(s, out);
 }
 
 // ---------------------------------------------------------------------------

@@ -5,6 +5,8 @@ advance nfiles x nsteps steps, writing the field files after every nsteps.
 
 Output directory layout (the reference's):
   <out>/Grid.txt                i, x, dxi/dx, d2x/dxi2 per x node (src/init.cpp:73-76)
+  <out>/prof.txt                wall-normal profiles of calcAvgChan (src/init.cpp:150-208),
+                                rewritten at the start and after every file (src/cuda_main.cu:311)
   <out>/fields/{x,y,z}.bin      node coordinates (src/init.cpp:78-88)
   <out>/fields/<c>.%07d.bin     state after each file (c = r, u, v, w, e)
   <out>/fields/ctl.%07d.json    configuration and time-advance control of each file
@@ -41,6 +43,25 @@ def write_grid_txt(path, g):
     with open(path, "w") as f:
         for i, (x, xp, xpp) in enumerate(zip(g["x"], g["xp"], g["xpp"])):
             f.write(f"{i} {x:f} {xp:f} {xpp:f}\n")
+
+
+def write_prof_txt(path, prof):
+    cols = [prof[c] for c in ("x", "rm", "um", "vm", "wm", "em", "rf", "uf", "vf", "wf", "ef")]
+    with open(path, "w") as f:
+        for row in zip(*cols):
+            f.write("\t".join(f"{v:f}" for v in row) + "\n")
+
+
+def print_res(s, g):
+    """the resolution report of printRes (src/init.cpp:242-255)"""
+    ret = s.friction_reynolds()
+    x, y, z = g["x"], g["y"], g["z"]
+    mx = len(x)
+    print(f"The average friction Reynolds number is: \t {ret:f}")
+    print(f"wall-normal wall: \t {(x[0] + x[1]) / 2 * ret:f}")
+    print(f"wall-normal center: \t {(x[mx // 2 + 1] - x[mx // 2 - 1]) / 2 * ret:f}")
+    print(f"span-wise: \t {(y[1] - y[0]) * ret:f}")
+    print(f"stream-wise: \t {(z[1] - z[0]) * ret:f}", flush=True)
 
 
 def main(argv=None):
@@ -81,6 +102,8 @@ def main(argv=None):
     g = s.grid()
     write_grid_txt(os.path.join(a.out, "Grid.txt"), g)
     s.write_grid(fdir)
+    print_res(s, g)
+    write_prof_txt(os.path.join(a.out, "prof.txt"), s.channel_profiles())
     start = max(a.restart, 0)
     if a.restart < 0:
         ff.save_checkpoint(s, fdir, 0)
@@ -90,6 +113,8 @@ def main(argv=None):
             kin, enst, tim = s.step(a.nsteps)
             wall = time.perf_counter() - t0
             ff.save_checkpoint(s, fdir, file)
+            write_prof_txt(os.path.join(a.out, "prof.txt"), s.channel_profiles())
+            st = s.flow_stats()
             base = (file - 1) * a.nsteps
             for t in range(0, a.nsteps - 1, c.check_bulk):
                 dt = tim[t + 1] - tim[t]
@@ -98,7 +123,8 @@ def main(argv=None):
             pts = c.mx * c.my * c.mz * a.nsteps / wall / 1e9
             last = next((i for i in range(a.nsteps - 1, -1, -1) if not math.isnan(kin[i])), a.nsteps - 1)
             print(f"file number: {file}  step: {file * a.nsteps}  time: {tim[-1]:f}  kin: {kin[last]:e}  "
-                  f"energy: {enst[last]:e}  ({pts:.3f} Gpt-steps/s)", flush=True)
+                  f"energy: {enst[last]:e}  kinetic: {st['kinetic']:e}  enstrophy: {st['enstrophy']:e}  "
+                  f"({pts:.3f} Gpt-steps/s)", flush=True)
     # index every field file present in the directory (earlier runs included)
     have = sorted({int(f[2:9]) for f in os.listdir(fdir) if f.startswith("r.") and f.endswith(".bin")})
     tmap = {}

@@ -210,6 +210,28 @@ class Solver:
         self._chk(self.L.cns_device_bytes(self.h, C.byref(b)), "cns_device_bytes")
         return b.value
 
+    PROFILE_COLUMNS = ("x", "rm", "um", "vm", "wm", "em", "rf", "uf", "vf", "wf", "ef")
+
+    def channel_profiles(self):
+        """wall-normal profiles of calcAvgChan (src/init.cpp:150-208): dict of
+        the prof.txt columns, each of length mx"""
+        out = np.empty(11 * self.config.mx)
+        self._chk(self.L.cns_channel_profiles(self.h, out), "cns_channel_profiles")
+        return dict(zip(self.PROFILE_COLUMNS, out.reshape(11, self.config.mx)))
+
+    def friction_reynolds(self):
+        """average friction Reynolds number of printRes (src/init.cpp:210-240)"""
+        r = C.c_double()
+        self._chk(self.L.cns_friction_reynolds(self.h, C.byref(r)), "cns_friction_reynolds")
+        return r.value
+
+    def flow_stats(self):
+        """kinetic energy and enstrophy (density-weighted means), mean squared
+        dilatation and mean density of the current state"""
+        out = np.empty(4)
+        self._chk(self.L.cns_flow_stats(self.h, out), "cns_flow_stats")
+        return dict(kinetic=out[0], enstrophy=out[1], dil2=out[2], rho=out[3])
+
     def write_fields(self, directory, timestep):
         """<directory>/{r,u,v,w,e}.<timestep:07d>.bin, float64 [mz][my][mx]
         (writeField, src/init.cpp:24-30)"""

@@ -127,6 +127,30 @@ int cns_set_sponge(cns_solver *s, const double *sigma_x, const double *sigma_z, 
 /* current dt, dp/dz and simulation time */
 int cns_get_scalars(cns_solver *s, double *dt, double *dpdz, double *time);
 
+/* Diagnostics of the current state, evaluated on the device in FP64 (FP32
+ * storage is read as stored value + low part + offset).
+ *
+ * cns_channel_profiles: the wall-normal profiles of calcAvgChan
+ * (src/init.cpp:150-208), averaged over y-z planes; prof receives 11*nx
+ * values, prof[c*nx + i] for the columns of its prof.txt: 0 x, 1 mean rho,
+ * 2-4 Favre means <rho u>/<rho>, <rho v>/<rho>, <rho w>/<rho>, 5 mean rhoE,
+ * 6-10 mean squared deviations of rho, u, v, w, rhoE from those means.
+ *
+ * cns_friction_reynolds: the average friction Reynolds number printRes
+ * prints (src/init.cpp:210-240) -- per y-z row, dw/dx at the first node from
+ * the advective-order stencil on w mirrored evenly about the bottom face,
+ * u_tau = sqrt(mu_w |dw/dx| / rho) with mu_w = 1/Re, averaged u_tau rho/mu_w.
+ *
+ * cns_flow_stats (no reference counterpart): out[0] kinetic energy
+ * int(rho |u|^2/2) / int(rho), out[1] enstrophy int(rho |curl u|^2/2) /
+ * int(rho), out[2] volume mean of (div u)^2, out[3] volume mean density;
+ * velocity derivatives are the RHS's viscous-order central differences and
+ * ghost rules, integrals use the cell widths in x.
+ * All three overwrite the first right-hand-side buffer (free between steps). */
+int cns_channel_profiles(cns_solver *s, double *prof);
+int cns_friction_reynolds(cns_solver *s, double *re_tau);
+int cns_flow_stats(cns_solver *s, double *out);
+
 /* Field files of writeField / initField (src/init.cpp:13-30), written and
  * read like saveFileMPI / readFileMPI (src/comm.cpp:205-279):
  * <dir>/<c>.<timestep, %07d>.bin for c = r, u, v, w, e, each the whole grid

@@ -168,5 +168,7 @@ def test_driver_restart_continues_the_run(tmp_path):
         lines = open(os.path.join(d, "solution.txt")).read().split("\n")
         assert [int(l.split()[0]) for l in lines if l] == [1, 3, 5, 7, 9, 11]
         assert len(open(os.path.join(d, "Grid.txt")).read().splitlines()) == 16
+        prof = np.loadtxt(os.path.join(d, "prof.txt"))
+        assert prof.shape == (16, 11)
     t3 = json.load(open(ff.ctl_path(os.path.join(a, "fields"), 3)))["control"]["time"]
     assert t3 > 0
