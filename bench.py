@@ -33,6 +33,7 @@ def parse_args():
     ap.add_argument("--order", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-storage run")
+    ap.add_argument("--no-tune", action="store_true", help="default launch shape instead of cns_tune")
     ap.add_argument("--err-steps", type=int, default=20, help="steps of the FP32-vs-FP64 error comparison")
     ap.add_argument("--ensemble-steps", type=int, default=100, help="steps of the 64-instance ensemble (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=100,
@@ -101,8 +102,9 @@ def copy_calibration(mib, reps=5):
 
 def ensemble_measure(cns, steps):
     """64 seeded Taylor-Green instances (32^3..128^3, Re 400-3200, Mach
-    0.1-0.5), 6th order, RK4, FP32 storage: each alone in turn, then all in
-    one tuned batch (one launch per RK stage, graph replay)."""
+    0.1-0.5), 6th order, RK4, FP32 storage: each alone in turn with its own
+    tuned launch shape (cns_tune), then all in one tuned batch (one launch
+    per RK stage, graph replay)."""
     from cudanavierstokes_amd.workloads import ensemble
     inst = ensemble(n_instances=64, seed=4, steps=steps)
     work = sum(i["config"].mx * i["config"].my * i["config"].mz for i in inst) * steps
@@ -110,6 +112,7 @@ def ensemble_measure(cns, steps):
     for i in inst:
         s = cns.Solver(i["config"])
         s.set_state(*s.taylor_green_fields())
+        s.tune()
         s.step(1, outputs=False)
         seq_ms += s.time_steps(steps)
         s.close()
@@ -130,7 +133,7 @@ def ensemble_measure(cns, steps):
     seq = work / (seq_ms / 1e3) / 1e9
     bat = work / (bat_ms / 1e3) / 1e9
     return {"workload": "64 Taylor-Green instances, sizes {32,48,64,96,128}^3 (seed 4), order 6, RK4, "
-                        f"FP32 storage, {steps} steps each",
+                        f"FP32 storage, {steps} steps each; sequential: each alone, launch shape tuned",
             "sequential_gpt_steps_per_s": round(seq, 4), "batched_gpt_steps_per_s": round(bat, 4),
             "speedup": round(bat / seq, 3), "sequential_ms": round(seq_ms, 2), "batched_ms": round(bat_ms, 2),
             "tune_s": round(tune_s, 3), "groups": groups}
@@ -171,6 +174,8 @@ def main():
         kernel (HIP events on the solver stream)."""
         sol = cns.Solver(cfg)
         sol.set_state(*sol.taylor_green_fields())
+        # launch shape: a few timed steps per pencil length, state restored
+        pencil = sol.tune() if not a.no_tune else None
         if a.warmup:
             sol.step(a.warmup, outputs=False)
         sol.sync()
@@ -185,7 +190,7 @@ def main():
         sol.close()
         per_launch_ms = kms / max(launches, 1)
         bytes_per_launch = RK4_WORDS * wsize * pts / 4.0
-        return {"ms": ms, "wall": wall, "value": pts * a.steps / (ms / 1e3) / 1e9,
+        return {"ms": ms, "wall": wall, "value": pts * a.steps / (ms / 1e3) / 1e9, "pencil": pencil,
                 "per_launch_ms": per_launch_ms, "launches": launches,
                 "achieved": bytes_per_launch / (per_launch_ms / 1e3) / 1e9,
                 "step_gbs": RK4_WORDS * wsize * pts * a.steps / (ms / 1e3) / 1e9}
@@ -233,7 +238,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(m64["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(m64["achieved"] / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_rhs (fused RHS + RK4 stage update)",
-                     "kernel_ms_per_launch": round(m64["per_launch_ms"], 4), "launches": m64["launches"]},
+                     "kernel_ms_per_launch": round(m64["per_launch_ms"], 4), "launches": m64["launches"],
+                     "pencil": m64["pencil"]},
         "step_hbm_frac": round(m64["step_gbs"] / HBM_PEAK_GBS, 4),
         "wall_s": round(m64["wall"], 3),
     }
@@ -247,7 +253,7 @@ def main():
             "speedup_vs_fp64": round(m32["value"] / m64["value"], 3),
             "roofline": {"bound": "hbm", "achieved": round(m32["achieved"], 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(m32["achieved"] / HBM_PEAK_GBS, 4),
-                         "kernel_ms_per_launch": round(m32["per_launch_ms"], 4)},
+                         "kernel_ms_per_launch": round(m32["per_launch_ms"], 4), "pencil": m32["pencil"]},
             "rel_err_vs_fp64": {"steps": a.err_steps, "l2_linf": fp32_error(a.err_steps)},
         }
     if a.ensemble_steps > 0:

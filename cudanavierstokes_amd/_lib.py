@@ -52,7 +52,7 @@ EXPORTS = [
     "cns_step", "cns_set_sponge", "cns_get_scalars", "cns_sync", "cns_profile_enable",
     "cns_profile_read", "cns_time_steps", "cns_device_bytes",
     "cns_write_fields", "cns_read_fields", "cns_write_grid", "cns_get_control", "cns_set_control",
-    "cns_channel_profiles", "cns_friction_reynolds", "cns_flow_stats",
+    "cns_channel_profiles", "cns_friction_reynolds", "cns_flow_stats", "cns_set_pencil", "cns_tune",
     "cns_batch_create", "cns_batch_tune", "cns_batch_step", "cns_batch_launch", "cns_batch_wait", "cns_batch_time_steps", "cns_batch_group_info", "cns_batch_set_shape",
     "cns_batch_groups", "cns_batch_destroy", "cns_batch_last_error",
 ]
@@ -104,6 +104,8 @@ def load(path=None):
     L.cns_channel_profiles.argtypes = [vp, dp]
     L.cns_friction_reynolds.argtypes = [vp, C.POINTER(C.c_double)]
     L.cns_flow_stats.argtypes = [vp, dp]
+    L.cns_set_pencil.argtypes = [vp, C.c_int]
+    L.cns_tune.argtypes = [vp, C.POINTER(C.c_int)]
     L.cns_batch_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(vp)]
     L.cns_batch_tune.argtypes = [vp]
     L.cns_batch_step.argtypes = [vp, C.c_int]

@@ -271,19 +271,37 @@ static void build_grid(cns_solver *s) {
                                    s->cVF[2 * SV - it] * xpp[i] * (xp[i] * xp[i] * xp[i]) * s->d_dx);
 }
 
+// Workgroup grid of the stage kernels: x-y tiles times z chunks of zc
+// planes.  Default chunk length (measured on 256^3 and 128^3 Taylor-Green
+// cases): the FP64 kernel (one workgroup per CU) prefers long marches, about
+// 512 workgroups, chunks of 8..128 planes; the FP32 kernel (two workgroups
+// per CU) prefers about 4096 workgroups; small grids want chunks of 8.
+// cns_tune measures instead.
+static void set_pencil(cns_solver *s, int zc) {
+    const cns_config &c = s->cfg;
+    s->zc = std::max(1, std::min(zc, c.nz));
+    s->nzc = (c.nz + s->zc - 1) / s->zc;
+    s->nblocks = s->ntx * s->nty * s->nzc;
+    s->base.zc = s->zc;
+    s->base.nzc = s->nzc;
+}
+
 static void launch_geometry(cns_solver *s) {
     const cns_config &c = s->cfg;
     s->ntx = (c.nx + TILE_X - 1) / TILE_X;
     s->nty = (c.ny + TILE_Y - 1) / TILE_Y;
-    const int cols = s->ntx * s->nty;
-    int want = (1024 + cols - 1) / cols;  // aim at ~1024 workgroups
-    int maxc = c.nz / (2 * s->S);         // keep chunks longer than the window
-    if (maxc < 1) maxc = 1;
-    if (want > maxc) want = maxc;
-    if (want < 1) want = 1;
-    s->zc = (c.nz + want - 1) / want;
-    s->nzc = (c.nz + s->zc - 1) / s->zc;
-    s->nblocks = s->ntx * s->nty * s->nzc;
+    const long cols = (long)s->ntx * s->nty;
+    const bool fp32 = c.precision == CNS_FP32;
+    const long target = fp32 ? 4096 : 512;
+    long zc = ((long)c.nz * cols + target - 1) / target;
+    zc = std::max<long>(zc, 8);
+    zc = std::min<long>(zc, 128);
+    // CNS_PENCIL=<planes>: z-chunk length override
+    if (const char *pz = std::getenv("CNS_PENCIL")) {
+        const int v = std::atoi(pz);
+        if (v >= 1) zc = v;
+    }
+    set_pencil(s, (int)zc);
 }
 
 static void fill_params(cns_solver *s) {
@@ -1097,6 +1115,74 @@ extern "C" int cns_step(cns_solver *s, int nsteps, double *kin, double *enst, do
     return CNS_OK;
 }
 
+extern "C" int cns_set_pencil(cns_solver *s, int pencil) {
+    if (!s) return fail(nullptr, CNS_ERR_ARG, "null solver");
+    if (s->in_batch) return fail(s, CNS_ERR_STATE, "cns_set_pencil: the solver is advanced by a batch");
+    if (pencil < 1) return fail(s, CNS_ERR_ARG, "cns_set_pencil: pencil must be >= 1");
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    set_pencil(s, pencil);
+    return CNS_OK;
+}
+
+extern "C" int cns_tune(cns_solver *s, int *pencil) {
+    if (!s) return fail(nullptr, CNS_ERR_ARG, "null solver");
+    if (!s->have_state) return fail(s, CNS_ERR_STATE, "cns_tune: no state set");
+    if (s->in_batch) return fail(s, CNS_ERR_STATE, "cns_tune: the solver is advanced by a batch");
+    HIPCHK(s, hipSetDevice(s->dev));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    // snapshot of everything a step changes: the state set (and its low
+    // part), the device scalars and the host step control
+    const size_t bytes = 5 * (size_t)s->n * s->esz;
+    const int cur0 = s->cur;
+    const long bsteps0 = s->bsteps;
+    const bool vfs0 = s->visc_from_state;
+    std::vector<char> hq(bytes), hl(s->comp ? bytes : 0);
+    Scalars sc0{};
+    HIPCHK(s, hipMemcpy(hq.data(), s->sets[cur0], bytes, hipMemcpyDeviceToHost));
+    if (s->comp) HIPCHK(s, hipMemcpy(hl.data(), s->lo[cur0], bytes, hipMemcpyDeviceToHost));
+    HIPCHK(s, hipMemcpy(&sc0, s->d_sc, sizeof(sc0), hipMemcpyDeviceToHost));
+    std::vector<int> cand;
+    for (int L : {8, 16, 32, 64, 128, 256})
+        if (L <= s->cfg.nz) cand.push_back(L);
+    if (std::find(cand.begin(), cand.end(), s->zc) == cand.end()) cand.push_back(s->zc);
+    const int steps = (int)std::max(1L, std::min(4L, 200000000L / s->n));
+    hipEvent_t a, b;
+    HIPCHK(s, hipEventCreate(&a));
+    HIPCHK(s, hipEventCreate(&b));
+    int best = s->zc, rc = CNS_OK;
+    float bt = 1e30f;
+    for (int L : cand) {
+        set_pencil(s, L);
+        if ((rc = queue_steps(s, 1, false, false))) break;
+        if (hipEventRecord(a, s->stream) != hipSuccess) break;
+        if ((rc = queue_steps(s, steps, false, false))) break;
+        float t = 0;
+        if (hipEventRecord(b, s->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+            hipEventElapsedTime(&t, a, b) != hipSuccess) {
+            rc = fail(s, CNS_ERR_HIP, "cns_tune: event timing failed");
+            break;
+        }
+        if (t < bt) {
+            bt = t;
+            best = L;
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    // restore the snapshot, bit for bit
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    HIPCHK(s, hipMemcpy(s->sets[cur0], hq.data(), bytes, hipMemcpyHostToDevice));
+    if (s->comp) HIPCHK(s, hipMemcpy(s->lo[cur0], hl.data(), bytes, hipMemcpyHostToDevice));
+    HIPCHK(s, hipMemcpy(s->d_sc, &sc0, sizeof(sc0), hipMemcpyHostToDevice));
+    s->cur = cur0;
+    s->bsteps = bsteps0;
+    s->visc_from_state = vfs0;
+    set_pencil(s, best);
+    if (rc) return rc;
+    if (pencil) *pencil = best;
+    return CNS_OK;
+}
+
 extern "C" int cns_time_steps(cns_solver *s, int nsteps, double *ms) {
     if (!s || !ms) return fail(s, CNS_ERR_ARG, "cns_time_steps: null argument");
     if (!s->have_state) return fail(s, CNS_ERR_STATE, "cns_time_steps: no state set");
@@ -1572,7 +1658,8 @@ template <typename R>
 static int queue_stage(cns_batch *b, const BatchGroup &g, int st) {
     const int nm = (int)g.m.size();
     const Job<R> *jobs = static_cast<const Job<R> *>(g.d_jobs) + (size_t)st * nm;
-    const int grid = std::max(1, std::min(g.grid, g.nwork));
+    // grid 0: one workgroup per work item (the dispatcher balances)
+    const int grid = g.grid > 0 ? std::max(1, std::min(g.grid, g.nwork)) : std::max(1, g.nwork);
     BCHK(b, launch_dil_batch<R>(g.SV, g.wallx, g.nonuni, grid, b->stream, jobs, nm, g.nwork, g.remap));
     BLAUNCHED(b, "k_dil_batch");
     BCHK(b, launch_rhs_batch<R>(g.S, g.SV, g.wallx, g.nonuni, g.unitv, grid, b->stream, jobs, nm, g.nwork,
@@ -1799,8 +1886,8 @@ extern "C" int cns_batch_tune(cns_batch *b) {
         for (int L : pencils) {
             if (L != bestL && L != 2 * bestL && 2 * L != bestL) continue;
             if ((rc = upload(b, g, L, true))) return rc;
-            for (int f : {1, 2, 4, 8}) {
-                g.grid = f * b->cus;
+            for (int f : {1, 2, 4, 8, 0}) {
+                g.grid = f * b->cus;  // 0: one workgroup per item
                 float t;
                 if ((rc = time_stage_any(b, g, t))) return rc;
                 if (t < bt) {
@@ -1910,7 +1997,7 @@ extern "C" int cns_batch_time_steps(cns_batch *b, int nsteps, double *ms) {
 
 extern "C" int cns_batch_set_shape(cns_batch *b, int g, int pencil, int grid, int remap) {
     if (!b) return bfail(nullptr, CNS_ERR_ARG, "null batch");
-    if (g < 0 || g >= (int)b->groups.size() || pencil < 1 || grid < 1)
+    if (g < 0 || g >= (int)b->groups.size() || pencil < 1 || grid < 0)
         return bfail(b, CNS_ERR_ARG, "cns_batch_set_shape: bad group, pencil or grid");
     if (b->pending) {
         const int rc = cns_batch_wait(b);

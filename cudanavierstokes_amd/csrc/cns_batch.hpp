@@ -39,32 +39,49 @@ struct Job {
     int nwork;     // work items (tiles x chunks) of this instance
 };
 
-// A pointer read from device memory is generic to the compiler; rebuilt as
-// an offset from a kernel-argument pointer (which the compiler knows to be
-// global) its loads and stores use global instead of flat instructions.
+// Job arrays are read through the constant address space: nothing in a
+// stage launch writes them (nor the step counters, which only the control
+// kernels between stage launches advance), so every read is a scalar load
+// and the values live in scalar registers -- as the tiles' buffer resources
+// require (a field pointer the compiler took for divergent would need a
+// loop over its distinct values around every buffer access).
 template <typename T>
-__device__ __forceinline__ T *glob(const void *base, T *p) {
-    const char *b = static_cast<const char *>(base);
-    return (T *)(b + ((const char *)p - b));
+using cptr = const __attribute__((address_space(4))) T *;
+
+template <typename T>
+__device__ __forceinline__ cptr<T> as_const(const T *p) {
+    return (cptr<T>)p;
 }
 
-// The stencil loops address everything from qin and dil, so those two are
-// rebuilt.  Done for FP64 storage only: in the FP32 kernel (two waves per
-// SIMD, at most 256 VGPRs) the extra 64-bit address arithmetic spills, which
-// costs more than the flat loads it removes.
+// stage arrays of a job for step parity par
 template <typename R>
-__device__ __forceinline__ StagePtrs<R> stage_ptrs(const void *base, const StagePtrs<R> &a) {
-    StagePtrs<R> g = a;
-    if (sizeof(R) == 8) {
-        g.qin = glob(base, a.qin);
-        g.dil = glob(base, a.dil);
-    }
+__device__ __forceinline__ StagePtrs<R> stage_arrays(cptr<Job<R>> J, int par) {
+    StagePtrs<R> g;
+    g.qin = J->A[par].qin;
+    g.dil = J->A[par].dil;
+    g.s1 = J->A[par].s1;
+    g.ra = J->A[par].ra;
+    g.rb = J->A[par].rb;
+    g.s2 = J->A[par].s2;
+    g.qout = J->A[par].qout;
+    g.rout = J->A[par].rout;
+    g.accout = J->A[par].accout;
+    g.s1lo = J->A[par].s1lo;
+    g.s2lo = J->A[par].s2lo;
+    g.qoutlo = J->A[par].qoutlo;
+    g.accoutlo = J->A[par].accoutlo;
+    g.fs = J->A[par].fs;
     return g;
+}
+
+template <typename R>
+__device__ __forceinline__ int step_parity(cptr<Job<R>> J) {
+    return as_const(J->ctl)->istep & 1;
 }
 
 // index of the job that owns work item w (jobs sorted by first)
 template <typename R>
-__device__ __forceinline__ int job_of(const Job<R> *__restrict__ jobs, int njobs, int w) {
+__device__ __forceinline__ int job_of(cptr<Job<R>> jobs, int njobs, int w) {
     int lo = 0, hi = njobs - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -83,12 +100,12 @@ __device__ __forceinline__ int job_of(const Job<R> *__restrict__ jobs, int njobs
 template <typename R, int S, int SV, int TX, int TY, bool WALLX, bool NONUNI, bool UNITV>
 __global__ __launch_bounds__(TX *TY, (rhs_waves<R, UNITV>())) void k_rhs_batch(const Job<R> *__restrict__ jobs, int njobs, int nwork,
                                                       int remap) {
+    const cptr<Job<R>> cj = as_const(jobs);
     for (int w0 = blockIdx.x; w0 < nwork; w0 += gridDim.x) {
         const int w = remap_block(w0, nwork, remap);
-        const int jb = job_of(jobs, njobs, w);
-        const Job<R> &J = jobs[jb];
-        const int par = J.ctl->istep & 1;
-        rhs_tile<R, S, SV, TX, TY, WALLX, NONUNI, UNITV>(J.P, stage_ptrs(jobs, J.A[par]), w - J.first);
+        const cptr<Job<R>> J = cj + job_of(cj, njobs, w);
+        const Params P = *(const Params *)&J->P;  // loads stay in the constant space
+        rhs_tile<R, S, SV, TX, TY, WALLX, NONUNI, UNITV>(P, stage_arrays(J, step_parity(J)), w - J->first);
         __syncthreads();  // LDS reuse by the next item
     }
 }
@@ -96,12 +113,12 @@ __global__ __launch_bounds__(TX *TY, (rhs_waves<R, UNITV>())) void k_rhs_batch(c
 template <typename R, int SV, int TX, int TY, bool WALLX, bool NONUNI>
 __global__ __launch_bounds__(TX *TY) void k_dil_batch(const Job<R> *__restrict__ jobs, int njobs, int nwork,
                                                       int remap) {
+    const cptr<Job<R>> cj = as_const(jobs);
     for (int w0 = blockIdx.x; w0 < nwork; w0 += gridDim.x) {
         const int w = remap_block(w0, nwork, remap);
-        const int jb = job_of(jobs, njobs, w);
-        const Job<R> &J = jobs[jb];
-        const int par = J.ctl->istep & 1;
-        dil_tile<R, SV, TX, TY, WALLX, NONUNI>(J.P, stage_ptrs(jobs, J.A[par]), glob(jobs, J.dil), w - J.first);
+        const cptr<Job<R>> J = cj + job_of(cj, njobs, w);
+        const Params P = *(const Params *)&J->P;  // loads stay in the constant space
+        dil_tile<R, SV, TX, TY, WALLX, NONUNI>(P, stage_arrays(J, step_parity(J)), J->dil, w - J->first);
         __syncthreads();
     }
 }

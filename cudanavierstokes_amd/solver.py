@@ -205,6 +205,17 @@ class Solver:
         self._chk(self.L.cns_time_steps(self.h, nsteps, C.byref(ms)), "cns_time_steps")
         return ms.value
 
+    def set_pencil(self, pencil):
+        """z planes per workgroup march of the stage kernels"""
+        self._chk(self.L.cns_set_pencil(self.h, int(pencil)), "cns_set_pencil")
+
+    def tune(self):
+        """time a few steps per pencil length, keep the fastest; the state
+        is restored bit for bit.  Returns the chosen length."""
+        p = C.c_int()
+        self._chk(self.L.cns_tune(self.h, C.byref(p)), "cns_tune")
+        return p.value
+
     def device_bytes(self):
         b = C.c_uint64()
         self._chk(self.L.cns_device_bytes(self.h, C.byref(b)), "cns_device_bytes")

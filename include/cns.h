@@ -197,6 +197,15 @@ int cns_profile_read(cns_solver *s, double *total_ms, int *launches);
  * solver's stream; *ms receives the elapsed time of the n_steps. */
 int cns_time_steps(cns_solver *s, int n_steps, double *ms);
 
+/* Launch shape of the stage kernels: each workgroup marches a pencil of
+ * this many z planes (1..nz; results do not depend on it).  cns_tune times
+ * a few steps per candidate length (8..256) on the current state, keeps the
+ * fastest and restores the state, time and step control bit for bit;
+ * *pencil (may be NULL) receives the choice.  No reference counterpart (the
+ * reference's z chunking, nDivZ, is a compile-time constant). */
+int cns_set_pencil(cns_solver *s, int pencil);
+int cns_tune(cns_solver *s, int *pencil);
+
 /* bytes of device memory held by the solver */
 int cns_device_bytes(const cns_solver *s, uint64_t *bytes);
 
@@ -237,8 +246,9 @@ int cns_batch_time_steps(cns_batch *b, int n_steps, double *ms);
 int cns_batch_group_info(const cns_batch *b, int g, int *n_members, int *pencil, int *grid, int *remap,
                          int *work_items);
 int cns_batch_groups(const cns_batch *b);
-/* set group g's launch shape by hand (pencil length >= 1, workgroups >= 1,
- * XCD-contiguous order 0/1) instead of cns_batch_tune */
+/* set group g's launch shape by hand (pencil length >= 1, persistent
+ * workgroups >= 1 or 0 for one workgroup per work item, XCD-contiguous order
+ * 0/1) instead of cns_batch_tune */
 int cns_batch_set_shape(cns_batch *b, int g, int pencil, int grid, int remap);
 int cns_batch_destroy(cns_batch *b);
 const char *cns_batch_last_error(const cns_batch *b);
