@@ -831,9 +831,12 @@ extern "C" int cns_get_state(cns_solver *s, double *r, double *u, double *v, dou
         return CNS_OK;
     }
     std::vector<float> hi(5 * (size_t)n), lo(5 * (size_t)n, 0.0f);
+    // on the solver's own stream: other streams' work (a batch) keeps running
+    HIPCHK(s, hipMemcpyAsync(hi.data(), s->sets[s->cur], hi.size() * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    if (s->comp)
+        HIPCHK(s, hipMemcpyAsync(lo.data(), s->lo[s->cur], lo.size() * sizeof(float), hipMemcpyDeviceToHost,
+                                 s->stream));
     HIPCHK(s, hipStreamSynchronize(s->stream));
-    HIPCHK(s, hipMemcpy(hi.data(), s->sets[s->cur], hi.size() * sizeof(float), hipMemcpyDeviceToHost));
-    if (s->comp) HIPCHK(s, hipMemcpy(lo.data(), s->lo[s->cur], lo.size() * sizeof(float), hipMemcpyDeviceToHost));
     host_parallel(n, [&](long a, long b) {
         for (long g = a; g < b; g++) {
             double x[5];

@@ -6,8 +6,9 @@ footprint of the admitted states stays within a budget; admitted states stay
 resident until their last step.  Every tick advances all admitted requests
 together with one batch (cns_batch_*: one launch per RK stage per kernel
 group, one replayed graph per step), by the smallest remaining step count or
-the tick length, whichever is less; finished requests leave, waiting ones are
-admitted, and the batch is rebuilt for the new membership.  Each request's
+the tick length, whichever is less; finished requests leave (their results
+are read back while the next tick runs), waiting ones are admitted, and the
+batch is rebuilt for the new membership.  Each request's
 result equals its lone cns_step result to rounding (the batch keeps every
 member's step count, so the CFL checks fall on the same steps).
 
@@ -100,7 +101,33 @@ class Server:
                 joining[r["id"]] = dict(req=r, solver=s, left=int(r["steps"]), admitted=clock(), bytes=need)
                 used[0] += need
 
-        while pending or waiting or active or joining:
+        finishing = []  # done on the GPU; results are read while the next tick runs
+
+        def finish():
+            tf = time.perf_counter()
+            for a in finishing:
+                r = a["req"]
+                state = a["solver"].get_state()
+                t_done = clock()
+                rec = dict(id=r["id"], kind=r["kind"], arrival=r["arrival"], admitted=a["admitted"],
+                           done=t_done, tts=t_done - r["arrival"], steps=int(r["steps"]),
+                           points=points(r["config"]))
+                if self.keep:
+                    rec["state"] = state
+                if self.ckpt is not None:
+                    ts = int(r["steps"]) + (int(r["restart"]["timestep"]) if "restart" in r else 0)
+                    d = os.path.join(self.ckpt, str(r["id"]))
+                    save_checkpoint(a["solver"], d, ts)
+                    rec["checkpoint"] = dict(dir=d, timestep=ts)
+                done.append(rec)
+                a["solver"].close()
+                used[0] -= a["bytes"]
+                if log:
+                    log(rec)
+            finishing.clear()
+            cost["finish_s"] += time.perf_counter() - tf
+
+        while pending or waiting or active or joining or finishing:
             admit()
             if joining:
                 active.update(joining)
@@ -109,6 +136,9 @@ class Server:
                     batch.close()
                     batch = None
             if not active:
+                if finishing:
+                    finish()  # frees memory: admit again before waiting
+                    continue
                 if pending:
                     wait = pending[0]["arrival"] - clock()
                     if wait > 0:
@@ -124,40 +154,21 @@ class Server:
             k = min(self.tick, min(a["left"] for a in active.values()))
             tb = time.perf_counter()
             batch.launch(k)
-            admit()  # host work for newcomers overlaps the batch on the GPU
+            # host work overlaps the batch on the GPU: results of the requests
+            # that finished at the last tick, then newcomers
+            finish()
+            admit()
             batch.wait()
             busy += time.perf_counter() - tb
             ticks += 1
-            finished = []
-            for rid, a in active.items():
+            finished = [rid for rid, a in active.items() if a["left"] == k]
+            for a in active.values():
                 a["left"] -= k
-                if a["left"] == 0:
-                    finished.append(rid)
             if finished:
-                tf = time.perf_counter()
                 batch.close()
                 batch = None
                 for rid in finished:
-                    a = active.pop(rid)
-                    state = a["solver"].get_state()
-                    t_done = clock()
-                    r = a["req"]
-                    rec = dict(id=r["id"], kind=r["kind"], arrival=r["arrival"], admitted=a["admitted"],
-                               done=t_done, tts=t_done - r["arrival"], steps=int(r["steps"]),
-                               points=points(r["config"]))
-                    if self.keep:
-                        rec["state"] = state
-                    if self.ckpt is not None:
-                        ts = int(r["steps"]) + (int(r["restart"]["timestep"]) if "restart" in r else 0)
-                        d = os.path.join(self.ckpt, str(r["id"]))
-                        save_checkpoint(a["solver"], d, ts)
-                        rec["checkpoint"] = dict(dir=d, timestep=ts)
-                    done.append(rec)
-                    a["solver"].close()
-                    used[0] -= a["bytes"]
-                    if log:
-                        log(rec)
-                cost["finish_s"] += time.perf_counter() - tf
+                    finishing.append(active.pop(rid))
         if batch is not None:
             batch.close()
         wall = clock()
