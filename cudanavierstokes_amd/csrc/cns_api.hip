@@ -1665,7 +1665,9 @@ static int queue_stage(cns_batch *b, const BatchGroup &g, int st) {
     const int grid = g.grid > 0 ? std::max(1, std::min(g.grid, g.nwork)) : std::max(1, g.nwork);
     BCHK(b, launch_dil_batch<R>(g.SV, g.wallx, g.nonuni, grid, b->stream, jobs, nm, g.nwork, g.remap));
     BLAUNCHED(b, "k_dil_batch");
-    BCHK(b, launch_rhs_batch<R>(g.S, g.SV, g.wallx, g.nonuni, g.unitv, grid, b->stream, jobs, nm, g.nwork,
+    // three update operands only in the last Kutta stage
+    const bool k3 = g.scheme == CNS_RK3_KUTTA && st == 2;
+    BCHK(b, launch_rhs_batch<R>(g.S, g.SV, g.wallx, g.nonuni, g.unitv, k3, grid, b->stream, jobs, nm, g.nwork,
                                 g.remap));
     BLAUNCHED(b, "k_rhs_batch");
     return CNS_OK;

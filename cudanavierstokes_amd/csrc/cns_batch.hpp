@@ -97,7 +97,7 @@ __device__ __forceinline__ int job_of(cptr<Job<R>> jobs, int njobs, int w) {
 // remapped list; with G a multiple of 8 one XCD works through a contiguous
 // range of items (neighbouring tiles of one instance share halo planes in
 // that XCD's L2).
-template <typename R, int S, int SV, int TX, int TY, bool WALLX, bool NONUNI, bool UNITV>
+template <typename R, int S, int SV, int TX, int TY, bool WALLX, bool NONUNI, bool UNITV, bool K3>
 __global__ __launch_bounds__(TX *TY, (rhs_waves<R, UNITV>())) void k_rhs_batch(const Job<R> *__restrict__ jobs, int njobs, int nwork,
                                                       int remap) {
     const cptr<Job<R>> cj = as_const(jobs);
@@ -105,7 +105,7 @@ __global__ __launch_bounds__(TX *TY, (rhs_waves<R, UNITV>())) void k_rhs_batch(c
         const int w = remap_block(w0, nwork, remap);
         const cptr<Job<R>> J = cj + job_of(cj, njobs, w);
         const Params P = *(const Params *)&J->P;  // loads stay in the constant space
-        rhs_tile<R, S, SV, TX, TY, WALLX, NONUNI, UNITV>(P, stage_arrays(J, step_parity(J)), w - J->first);
+        rhs_tile<R, S, SV, TX, TY, WALLX, NONUNI, UNITV, K3>(P, stage_arrays(J, step_parity(J)), w - J->first);
         __syncthreads();  // LDS reuse by the next item
     }
 }

@@ -12,27 +12,32 @@
 
 namespace cns {
 
-template <typename R, int S, int SV, bool UNITV>
+template <typename R, int S, int SV, bool UNITV, bool K3>
 static hipError_t go_rhs_v(bool wallx, bool nonuni, int nb, hipStream_t st, const Params &P, const StagePtrs<R> &A) {
     const dim3 g(nb), b(TILE_X * TILE_Y);
     if (wallx) {
         if (nonuni)
-            hipLaunchKernelGGL((k_rhs<R, S, SV, TILE_X, TILE_Y, true, true, UNITV>), g, b, 0, st, P, A);
+            hipLaunchKernelGGL((k_rhs<R, S, SV, TILE_X, TILE_Y, true, true, UNITV, K3>), g, b, 0, st, P, A);
         else
-            hipLaunchKernelGGL((k_rhs<R, S, SV, TILE_X, TILE_Y, true, false, UNITV>), g, b, 0, st, P, A);
+            hipLaunchKernelGGL((k_rhs<R, S, SV, TILE_X, TILE_Y, true, false, UNITV, K3>), g, b, 0, st, P, A);
     } else {
         if (nonuni)
-            hipLaunchKernelGGL((k_rhs<R, S, SV, TILE_X, TILE_Y, false, true, UNITV>), g, b, 0, st, P, A);
+            hipLaunchKernelGGL((k_rhs<R, S, SV, TILE_X, TILE_Y, false, true, UNITV, K3>), g, b, 0, st, P, A);
         else
-            hipLaunchKernelGGL((k_rhs<R, S, SV, TILE_X, TILE_Y, false, false, UNITV>), g, b, 0, st, P, A);
+            hipLaunchKernelGGL((k_rhs<R, S, SV, TILE_X, TILE_Y, false, false, UNITV, K3>), g, b, 0, st, P, A);
     }
     return hipGetLastError();
 }
 
 template <typename R, int S, int SV>
 static hipError_t go_rhs(bool wallx, bool nonuni, int nb, hipStream_t st, const Params &P, const StagePtrs<R> &A) {
-    return P.unit_visc ? go_rhs_v<R, S, SV, true>(wallx, nonuni, nb, st, P, A)
-                       : go_rhs_v<R, S, SV, false>(wallx, nonuni, nb, st, P, A);
+    // three update operands only in the last Kutta stage (q0, k1, k2)
+    const bool k3 = P.use_ra && P.use_rb;
+    if (P.unit_visc)
+        return k3 ? go_rhs_v<R, S, SV, true, true>(wallx, nonuni, nb, st, P, A)
+                  : go_rhs_v<R, S, SV, true, false>(wallx, nonuni, nb, st, P, A);
+    return k3 ? go_rhs_v<R, S, SV, false, true>(wallx, nonuni, nb, st, P, A)
+              : go_rhs_v<R, S, SV, false, false>(wallx, nonuni, nb, st, P, A);
 }
 
 template <typename R, int SV>
@@ -53,33 +58,36 @@ static hipError_t go_dil(bool wallx, bool nonuni, int nb, hipStream_t st, const 
     return hipGetLastError();
 }
 
-template <typename R, int S, int SV, bool UNITV>
+template <typename R, int S, int SV, bool UNITV, bool K3>
 static hipError_t go_rhs_batch_v(bool wallx, bool nonuni, int grid, hipStream_t st, const Job<R> *jobs, int njobs,
                                  int nwork, int remap) {
     const dim3 g(grid), b(TILE_X * TILE_Y);
     if (wallx) {
         if (nonuni)
-            hipLaunchKernelGGL((k_rhs_batch<R, S, SV, TILE_X, TILE_Y, true, true, UNITV>), g, b, 0, st, jobs, njobs,
+            hipLaunchKernelGGL((k_rhs_batch<R, S, SV, TILE_X, TILE_Y, true, true, UNITV, K3>), g, b, 0, st, jobs, njobs,
                                nwork, remap);
         else
-            hipLaunchKernelGGL((k_rhs_batch<R, S, SV, TILE_X, TILE_Y, true, false, UNITV>), g, b, 0, st, jobs, njobs,
+            hipLaunchKernelGGL((k_rhs_batch<R, S, SV, TILE_X, TILE_Y, true, false, UNITV, K3>), g, b, 0, st, jobs, njobs,
                                nwork, remap);
     } else {
         if (nonuni)
-            hipLaunchKernelGGL((k_rhs_batch<R, S, SV, TILE_X, TILE_Y, false, true, UNITV>), g, b, 0, st, jobs, njobs,
+            hipLaunchKernelGGL((k_rhs_batch<R, S, SV, TILE_X, TILE_Y, false, true, UNITV, K3>), g, b, 0, st, jobs, njobs,
                                nwork, remap);
         else
-            hipLaunchKernelGGL((k_rhs_batch<R, S, SV, TILE_X, TILE_Y, false, false, UNITV>), g, b, 0, st, jobs,
+            hipLaunchKernelGGL((k_rhs_batch<R, S, SV, TILE_X, TILE_Y, false, false, UNITV, K3>), g, b, 0, st, jobs,
                                njobs, nwork, remap);
     }
     return hipGetLastError();
 }
 
 template <typename R, int S, int SV>
-static hipError_t go_rhs_batch(bool wallx, bool nonuni, bool unitv, int grid, hipStream_t st, const Job<R> *jobs,
-                               int njobs, int nwork, int remap) {
-    return unitv ? go_rhs_batch_v<R, S, SV, true>(wallx, nonuni, grid, st, jobs, njobs, nwork, remap)
-                 : go_rhs_batch_v<R, S, SV, false>(wallx, nonuni, grid, st, jobs, njobs, nwork, remap);
+static hipError_t go_rhs_batch(bool wallx, bool nonuni, bool unitv, bool k3, int grid, hipStream_t st,
+                               const Job<R> *jobs, int njobs, int nwork, int remap) {
+    if (unitv)
+        return k3 ? go_rhs_batch_v<R, S, SV, true, true>(wallx, nonuni, grid, st, jobs, njobs, nwork, remap)
+                  : go_rhs_batch_v<R, S, SV, true, false>(wallx, nonuni, grid, st, jobs, njobs, nwork, remap);
+    return k3 ? go_rhs_batch_v<R, S, SV, false, true>(wallx, nonuni, grid, st, jobs, njobs, nwork, remap)
+              : go_rhs_batch_v<R, S, SV, false, false>(wallx, nonuni, grid, st, jobs, njobs, nwork, remap);
 }
 
 template <typename R, int SV>
@@ -139,20 +147,20 @@ hipError_t LAUNCH_DIL(bool wallx, bool nonuni, int nb, hipStream_t st, const Par
 }
 
 template <typename R>
-hipError_t LAUNCH_RHS_BATCH(int sv, bool wallx, bool nonuni, bool unitv, int grid, hipStream_t st, const Job<R> *jobs,
-                            int njobs, int nwork, int remap) {
+hipError_t LAUNCH_RHS_BATCH(int sv, bool wallx, bool nonuni, bool unitv, bool k3, int grid, hipStream_t st,
+                            const Job<R> *jobs, int njobs, int nwork, int remap) {
     switch (sv) {
 #if CNS_S >= 1
-    case 1: return go_rhs_batch<R, CNS_S, 1>(wallx, nonuni, unitv, grid, st, jobs, njobs, nwork, remap);
+    case 1: return go_rhs_batch<R, CNS_S, 1>(wallx, nonuni, unitv, k3, grid, st, jobs, njobs, nwork, remap);
 #endif
 #if CNS_S >= 2
-    case 2: return go_rhs_batch<R, CNS_S, 2>(wallx, nonuni, unitv, grid, st, jobs, njobs, nwork, remap);
+    case 2: return go_rhs_batch<R, CNS_S, 2>(wallx, nonuni, unitv, k3, grid, st, jobs, njobs, nwork, remap);
 #endif
 #if CNS_S >= 3
-    case 3: return go_rhs_batch<R, CNS_S, 3>(wallx, nonuni, unitv, grid, st, jobs, njobs, nwork, remap);
+    case 3: return go_rhs_batch<R, CNS_S, 3>(wallx, nonuni, unitv, k3, grid, st, jobs, njobs, nwork, remap);
 #endif
 #if CNS_S >= 4
-    case 4: return go_rhs_batch<R, CNS_S, 4>(wallx, nonuni, unitv, grid, st, jobs, njobs, nwork, remap);
+    case 4: return go_rhs_batch<R, CNS_S, 4>(wallx, nonuni, unitv, k3, grid, st, jobs, njobs, nwork, remap);
 #endif
     }
     return hipErrorInvalidValue;
@@ -168,7 +176,7 @@ hipError_t LAUNCH_DIL_BATCH(bool wallx, bool nonuni, int grid, hipStream_t st, c
 #define CNS_INSTANTIATE(R)                                                                                        \
     template hipError_t LAUNCH_RHS<R>(int, bool, bool, int, hipStream_t, const Params &, const StagePtrs<R> &);  \
     template hipError_t LAUNCH_DIL<R>(bool, bool, int, hipStream_t, const Params &, const StagePtrs<R> &, R *);  \
-    template hipError_t LAUNCH_RHS_BATCH<R>(int, bool, bool, bool, int, hipStream_t, const Job<R> *, int, int,    \
+    template hipError_t LAUNCH_RHS_BATCH<R>(int, bool, bool, bool, bool, int, hipStream_t, const Job<R> *, int, int, \
                                             int);                                                                 \
     template hipError_t LAUNCH_DIL_BATCH<R>(bool, bool, int, hipStream_t, const Job<R> *, int, int, int);
 

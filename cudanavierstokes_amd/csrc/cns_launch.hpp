@@ -17,8 +17,8 @@ constexpr int TILE_Y = 16;
     hipError_t launch_dil_s##N(bool wallx, bool nonuni, int nblocks, hipStream_t st, const Params &P,         \
                                const StagePtrs<R> &A, R *dil);                                                 \
     template <typename R>                                                                                      \
-    hipError_t launch_rhs_batch_s##N(int sv, bool wallx, bool nonuni, bool unitv, int grid, hipStream_t st,    \
-                                     const Job<R> *jobs, int njobs, int nwork, int remap);                    \
+    hipError_t launch_rhs_batch_s##N(int sv, bool wallx, bool nonuni, bool unitv, bool k3, int grid,           \
+                                     hipStream_t st, const Job<R> *jobs, int njobs, int nwork, int remap);     \
     template <typename R>                                                                                      \
     hipError_t launch_dil_batch_s##N(bool wallx, bool nonuni, int grid, hipStream_t st, const Job<R> *jobs,   \
                                      int njobs, int nwork, int remap);
@@ -54,13 +54,13 @@ inline hipError_t launch_dil(int sv, bool wallx, bool nonuni, int nblocks, hipSt
 }
 
 template <typename R>
-inline hipError_t launch_rhs_batch(int s, int sv, bool wallx, bool nonuni, bool unitv, int grid, hipStream_t st,
-                                   const Job<R> *jobs, int njobs, int nwork, int remap) {
+inline hipError_t launch_rhs_batch(int s, int sv, bool wallx, bool nonuni, bool unitv, bool k3, int grid,
+                                   hipStream_t st, const Job<R> *jobs, int njobs, int nwork, int remap) {
     switch (s) {
-    case 1: return launch_rhs_batch_s1<R>(sv, wallx, nonuni, unitv, grid, st, jobs, njobs, nwork, remap);
-    case 2: return launch_rhs_batch_s2<R>(sv, wallx, nonuni, unitv, grid, st, jobs, njobs, nwork, remap);
-    case 3: return launch_rhs_batch_s3<R>(sv, wallx, nonuni, unitv, grid, st, jobs, njobs, nwork, remap);
-    case 4: return launch_rhs_batch_s4<R>(sv, wallx, nonuni, unitv, grid, st, jobs, njobs, nwork, remap);
+    case 1: return launch_rhs_batch_s1<R>(sv, wallx, nonuni, unitv, k3, grid, st, jobs, njobs, nwork, remap);
+    case 2: return launch_rhs_batch_s2<R>(sv, wallx, nonuni, unitv, k3, grid, st, jobs, njobs, nwork, remap);
+    case 3: return launch_rhs_batch_s3<R>(sv, wallx, nonuni, unitv, k3, grid, st, jobs, njobs, nwork, remap);
+    case 4: return launch_rhs_batch_s4<R>(sv, wallx, nonuni, unitv, k3, grid, st, jobs, njobs, nwork, remap);
     }
     return hipErrorInvalidValue;
 }

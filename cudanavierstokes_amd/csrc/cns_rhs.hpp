@@ -301,11 +301,6 @@ __device__ __forceinline__ void block_max_atomic(double v, unsigned long long *o
     }
 }
 
-template <typename R>
-__device__ __forceinline__ void phase_fence() {
-    if (sizeof(R) == 8) __builtin_amdgcn_sched_barrier(0);
-}
-
 // Bits of the stage-update flags.
 enum { F_Q = 1, F_RA = 2, F_RB = 4, F_ACC = 8, F_R = 16, F_SPONGE = 32, F_VISC = 64, F_COMP = 128, F_QLO = 256 };
 
@@ -329,7 +324,7 @@ __device__ __forceinline__ long long uniform64(long long v) {
 // With FP32 storage (R = float) the state is stored as offsets from the
 // reference state and the update is evaluated in FP64 on the compensated
 // pair hi + lo, rounded back to hi and lo (F_COMP).
-template <typename R, int S, int SV, int TX, int TY, bool WALLX, bool NONUNI, bool UNITV>
+template <typename R, int S, int SV, int TX, int TY, bool WALLX, bool NONUNI, bool UNITV, bool K3>
 __device__ __forceinline__ void rhs_tile(const Params &P, const StagePtrs<R> &A, const int bb) {
     constexpr int H = S, LX = TX + 2 * H, LY = TY + 2 * H, NT = TX * TY, W = 2 * H + 1;
     constexpr int NL = UNITV ? NQ - 1 : NQ;  // staged quantities (mu rebuilt from T when mu = T/Re)
@@ -457,28 +452,36 @@ __device__ __forceinline__ void rhs_tile(const Params &P, const StagePtrs<R> &A,
     for (int k = z0; k < z1; ++k) {
         const int b = (k - z0) & 1;
         const long poff = (long)k * mxy;
-        // operands of this plane's update, three slots always loaded: s1,
-        // ra, and rb (Kutta) or s2 (RK4) -- never both; arrays a stage does
-        // not use alias the stage input, so their loads are cache hits
-        R e1[5], ea[5], e3[5], e1l[5], e3l[5];
+        // operands of this plane's update, always loaded: s1 and a second
+        // operand, ra (low-storage RK3, Kutta) or s2 (RK4's accumulator), and
+        // with K3 (the last Kutta stage) rb as a third; an array a stage does
+        // not use aliases the stage input, so its loads are cache hits
+        R e1[5], e2[5], e3[5], e1l[5], e2l[5];
         {
-            const R *p1 = arr(0, poff), *p2 = arr(1, poff), *p3 = arr((fl & F_RB) ? 2 : 3, poff);
+            const R *p1 = arr(0, poff), *p2 = arr((K3 || (fl & F_RA)) ? 1 : 3, poff);
 #pragma unroll
             for (int q = 0; q < 5; q++) {
                 e1[q] = ld_at(p1 + q * fs, coff);
-                ea[q] = ld_at(p2 + q * fs, coff);
-                e3[q] = ld_at(p3 + q * fs, coff);
+                e2[q] = ld_at(p2 + q * fs, coff);
+            }
+            if constexpr (K3) {
+                const R *p3 = arr(2, poff);
+#pragma unroll
+                for (int q = 0; q < 5; q++) e3[q] = ld_at(p3 + q * fs, coff);
+            } else {
+#pragma unroll
+                for (int q = 0; q < 5; q++) e3[q] = R(0);
             }
             if constexpr (Off<R>::on) {
-                const R *l1 = arr(7, poff), *l3 = arr(8, poff);
+                const R *l1 = arr(7, poff), *l2 = arr(8, poff);
 #pragma unroll
                 for (int q = 0; q < 5; q++) {
                     e1l[q] = ld_at(l1 + q * fs, coff);
-                    e3l[q] = ld_at(l3 + q * fs, coff);
+                    e2l[q] = K3 ? R(0) : ld_at(l2 + q * fs, coff);
                 }
             } else {
 #pragma unroll
-                for (int q = 0; q < 5; q++) e1l[q] = e3l[q] = R(0);
+                for (int q = 0; q < 5; q++) e1l[q] = e2l[q] = R(0);
             }
         }
         __builtin_amdgcn_sched_barrier(0);
@@ -516,29 +519,19 @@ __device__ __forceinline__ void rhs_tile(const Params &P, const StagePtrs<R> &A,
                 return (UNITV && q == QM) ? (Off<R>::on ? T0 + win[QT][H + a] : win[QT][H + a]) * invRe
                                           : win[q][H + a];
             };
-            // FP64: scheduling barriers between the phases keep the compiler
-            // from hoisting every LDS read of the point to the top (register
-            // pressure at one wave per SIMD); the FP32 kernel schedules
-            // better without them
             R d1x[3], d2x[3], d1y[3], d2y[3], d1z[3], d2z[3];
             vel_stencils<R, SV, 0, NONUNI>(P, FX, vs, xpi, d1x, d2x);
-            phase_fence<R>();
             vel_stencils<R, SV, 1, false>(P, FY, vs, xpi, d1y, d2y);
-            phase_fence<R>();
             vel_stencils<R, SV, 2, false>(P, FZ, vs, xpi, d1z, d2z);
-            phase_fence<R>();
             const R g[9] = {d1x[0], d1x[1], d1x[2], d1y[0], d1y[1], d1y[2], d1z[0], d1z[1], d1z[2]};
             // x part (=), y part (+=), z part (+=) with the forcing, as
             // src/cuda_rhs.cu:123-127,255-259,389-393 accumulate them
             R rhs[5], part[5];
             dir_terms<R, S, SV, 0, NONUNI, UNITV>(P, FX, vs, xpi, g, d2x, rhs);
-            phase_fence<R>();
             dir_terms<R, S, SV, 1, false, UNITV>(P, FY, vs, xpi, g, d2y, part);
 #pragma unroll
             for (int q = 0; q < 5; q++) rhs[q] = rhs[q] + part[q];
-            phase_fence<R>();
             dir_terms<R, S, SV, 2, false, UNITV>(P, FZ, vs, xpi, g, d2z, part);
-            phase_fence<R>();
             const R wc = win[QW][H];
             rhs[0] = rhs[0] + part[0];
             rhs[1] = rhs[1] + part[1];
@@ -577,8 +570,8 @@ __device__ __forceinline__ void rhs_tile(const Params &P, const StagePtrs<R> &A,
 #pragma unroll
                     for (int q = 0; q < 5; q++) {
                         double inc = 0.0;
-                        if (fl & F_RA) inc += s_w[0] * (double)ea[q];
-                        if (fl & F_RB) inc += s_w[1] * (double)e3[q];
+                        if (fl & F_RA) inc += s_w[0] * (double)e2[q];
+                        if (K3 && (fl & F_RB)) inc += s_w[1] * (double)e3[q];
                         inc += s_w[2] * (double)rhs[q];
                         const double x = ((double)e1[q] + (comp ? (double)e1l[q] : 0.0)) + dtd * inc;
                         const R hi = R(x);
@@ -597,7 +590,7 @@ __device__ __forceinline__ void rhs_tile(const Params &P, const StagePtrs<R> &A,
 #pragma unroll
                     for (int q = 0; q < 5; q++) {
                         const double x =
-                            ((double)e3[q] + (comp ? (double)e3l[q] : 0.0)) + dtd * (s_w[3] * (double)rhs[q]);
+                            ((double)e2[q] + (comp ? (double)e2l[q] : 0.0)) + dtd * (s_w[3] * (double)rhs[q]);
                         const R hi = R(x);
                         st_at(ao + q * fs, coff, hi);
                         if (comp) st_at(aol + q * fs, coff, R(x - (double)hi));
@@ -609,8 +602,8 @@ __device__ __forceinline__ void rhs_tile(const Params &P, const StagePtrs<R> &A,
 #pragma unroll
                     for (int q = 0; q < 5; q++) {
                         R inc = R(0);
-                        if (fl & F_RA) inc += R(s_w[0]) * ea[q];
-                        if (fl & F_RB) inc += R(s_w[1]) * e3[q];
+                        if (fl & F_RA) inc += R(s_w[0]) * e2[q];
+                        if (K3 && (fl & F_RB)) inc += R(s_w[1]) * e3[q];
                         inc += R(s_w[2]) * rhs[q];
                         st_at(qo + q * fs, coff, e1[q] + dt * inc);
                     }
@@ -623,7 +616,7 @@ __device__ __forceinline__ void rhs_tile(const Params &P, const StagePtrs<R> &A,
                 if (fl & F_ACC) {
                     R *const ao = arr(6, poff);
 #pragma unroll
-                    for (int q = 0; q < 5; q++) st_at(ao + q * fs, coff, e3[q] + dt * (R(s_w[3]) * rhs[q]));
+                    for (int q = 0; q < 5; q++) st_at(ao + q * fs, coff, e2[q] + dt * (R(s_w[3]) * rhs[q]));
                 }
             }
         }
@@ -644,9 +637,9 @@ constexpr int rhs_waves() {
     return (sizeof(R) == 4 && UNITV) ? 2 : 1;
 }
 
-template <typename R, int S, int SV, int TX, int TY, bool WALLX, bool NONUNI, bool UNITV>
+template <typename R, int S, int SV, int TX, int TY, bool WALLX, bool NONUNI, bool UNITV, bool K3>
 __global__ __launch_bounds__(TX *TY, (rhs_waves<R, UNITV>())) void k_rhs(Params P, StagePtrs<R> A) {
-    rhs_tile<R, S, SV, TX, TY, WALLX, NONUNI, UNITV>(P, A, remap_block(blockIdx.x, gridDim.x, P.xcd_remap));
+    rhs_tile<R, S, SV, TX, TY, WALLX, NONUNI, UNITV, K3>(P, A, remap_block(blockIdx.x, gridDim.x, P.xcd_remap));
 }
 
 // Dilatation of the stage input: du/dx + dv/dy + dw/dz at viscous order
