@@ -274,8 +274,9 @@ static void build_grid(cns_solver *s) {
 // Workgroup grid of the stage kernels: x-y tiles times z chunks of zc
 // planes.  Default chunk length (measured on 256^3 and 128^3 Taylor-Green
 // cases): the FP64 kernel (one workgroup per CU) prefers long marches, about
-// 512 workgroups, chunks of 8..128 planes; the FP32 kernel (two workgroups
-// per CU) prefers about 4096 workgroups; small grids want chunks of 8.
+// one workgroup per CU, chunks of 8..256 planes; the FP32 kernel (two
+// workgroups per CU) prefers about 4096 workgroups; small grids want chunks
+// of 8.
 // cns_tune measures instead.
 static void set_pencil(cns_solver *s, int zc) {
     const cns_config &c = s->cfg;
@@ -292,10 +293,10 @@ static void launch_geometry(cns_solver *s) {
     s->nty = (c.ny + TILE_Y - 1) / TILE_Y;
     const long cols = (long)s->ntx * s->nty;
     const bool fp32 = c.precision == CNS_FP32;
-    const long target = fp32 ? 4096 : 512;
+    const long target = fp32 ? 4096 : 256;
     long zc = ((long)c.nz * cols + target - 1) / target;
     zc = std::max<long>(zc, 8);
-    zc = std::min<long>(zc, 128);
+    zc = std::min<long>(zc, fp32 ? 128 : 256);
     // CNS_PENCIL=<planes>: z-chunk length override
     if (const char *pz = std::getenv("CNS_PENCIL")) {
         const int v = std::atoi(pz);
