@@ -34,6 +34,7 @@ def parse_args():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-storage run")
     ap.add_argument("--no-tune", action="store_true", help="default launch shape instead of cns_tune")
+    ap.add_argument("--other-steps", type=int, default=20, help="steps per other reference configuration (0: skip)")
     ap.add_argument("--err-steps", type=int, default=20, help="steps of the FP32-vs-FP64 error comparison")
     ap.add_argument("--ensemble-steps", type=int, default=100, help="steps of the 64-instance ensemble (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=100,
@@ -83,6 +84,32 @@ def cpu_baseline(n, order, threads):
     return {"value": n ** 3 / dt / 1e9, "unit": "Gpt-RKsteps/s", "cores": threads, "kind": "port",
             "sample": f"1 RK4 step of the {n}^3 order-{order} Taylor-Green case on {threads} host threads "
                       f"({dt:.2f} s)"}
+
+
+def other_configs(cns, steps):
+    """Throughput of the other reference configurations (steps with the
+    state resident, launch shape tuned): the 64^3 4th-order Taylor-Green case
+    with the low-storage RK3, and the 160x128x192 isothermal-wall channel at
+    6th order (x wall-normal, stretched; forcing on), FP64 and FP32 storage."""
+    out = []
+    cases = [("tgv64_o4_rk3_fp64", cns.taylor_green(n=64, order=4, scheme="rk3"), "tgv"),
+             ("channel160x128x192_o6_rk3_fp64", cns.channel(mx=160, my=128, mz=192, order=6, scheme="rk3"), "channel"),
+             ("channel160x128x192_o6_rk3_fp32",
+              cns.channel(mx=160, my=128, mz=192, order=6, scheme="rk3", precision=1), "channel")]
+    for name, cfg, kind in cases:
+        s = cns.Solver(cfg)
+        if kind == "tgv":
+            s.init_taylor_green()
+        else:
+            s.init_channel(seed=1)
+        pencil = s.tune()
+        s.step(2, outputs=False)
+        ms = s.time_steps(steps)
+        s.close()
+        n = cfg.mx * cfg.my * cfg.mz
+        out.append({"workload": name, "grid": [cfg.mx, cfg.my, cfg.mz], "steps": steps, "pencil": pencil,
+                    "ms_per_step": round(ms / steps, 4), "gpt_rksteps_per_s": round(n * steps / ms / 1e6, 4)})
+    return out
 
 
 def copy_calibration(mib, reps=5):
@@ -260,6 +287,8 @@ def main():
         out["ensemble"] = ensemble_measure(cns, a.ensemble_steps)
     if a.serve_requests > 0:
         out["serving"] = serve_measure(a.serve_requests)
+    if a.other_steps > 0:
+        out["other_configs"] = other_configs(cns, a.other_steps)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(n, order, a.cpu_threads or host_cores())
     print(json.dumps(out), flush=True)
