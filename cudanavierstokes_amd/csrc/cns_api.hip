@@ -1794,7 +1794,11 @@ extern "C" int cns_batch_create(cns_solver *const *members, int n, cns_batch **o
             g->unitv = unitv;
             g->scheme = m->cfg.scheme;
             g->nst = n_stages(m);
-            g->grid = 4 * b->cus;
+            // untuned shape: one workgroup per item (the dispatcher balances
+            // members of different sizes); short pencils for FP32 storage,
+            // whose kernels run two workgroups per CU
+            g->grid = 0;
+            g->pencil = fp32 ? 16 : 32;
         }
         g->m.push_back(m);
         g->forcing = g->forcing || m->cfg.forcing_z;
