@@ -455,9 +455,12 @@ __device__ __forceinline__ void rhs_tile(const Params &P, const StagePtrs<R> &A,
         // operands of this plane's update, always loaded: s1 and a second
         // operand, ra (low-storage RK3, Kutta) or s2 (RK4's accumulator), and
         // with K3 (the last Kutta stage) rb as a third; an array a stage does
-        // not use aliases the stage input, so its loads are cache hits
+        // not use aliases the stage input, so its loads are cache hits.
+        // Issued before the window feed and halo prefetches with FP32
+        // storage, after them (shorter register lifetimes) with FP64, which
+        // measured faster each way.
         R e1[5], e2[5], e3[5], e1l[5], e2l[5];
-        {
+        auto load_update_operands = [&]() {
             const R *p1 = arr(0, poff), *p2 = arr((K3 || (fl & F_RA)) ? 1 : 3, poff);
 #pragma unroll
             for (int q = 0; q < 5; q++) {
@@ -483,7 +486,9 @@ __device__ __forceinline__ void rhs_tile(const Params &P, const StagePtrs<R> &A,
 #pragma unroll
                 for (int q = 0; q < 5; q++) e1l[q] = e2l[q] = R(0);
             }
-        }
+        };
+        constexpr bool early_ops = sizeof(R) == 4;
+        if constexpr (early_ops) load_update_operands();
         __builtin_amdgcn_sched_barrier(0);
         {
             R t[NQ];
@@ -504,6 +509,7 @@ __device__ __forceinline__ void rhs_tile(const Params &P, const StagePtrs<R> &A,
             load_raw(A, (long)wrapi(k + 1, P.mz) * mxy, hoff, hr);
         }
         __syncthreads();
+        if constexpr (!early_ops) load_update_operands();
 
         if (act) {
             auto FX = [&](int q, int a) -> R {

@@ -198,7 +198,9 @@ int cns_profile_read(cns_solver *s, double *total_ms, int *launches);
 int cns_time_steps(cns_solver *s, int n_steps, double *ms);
 
 /* Launch shape of the stage kernels: each workgroup marches a pencil of
- * this many z planes (1..nz; results do not depend on it).  cns_tune times
+ * this many z planes (1..nz; results depend on it only through rounding:
+ * every point follows the same arithmetic, but the compiled loop copies may
+ * fuse multiply-adds differently).  cns_tune times
  * a few steps per candidate length (8..256) on the current state, keeps the
  * fastest and restores the state, time and step control bit for bit;
  * *pencil (may be NULL) receives the choice.  No reference counterpart (the
