@@ -221,13 +221,13 @@ __device__ __forceinline__ void dir_terms(const Params &P, Acc f, const double *
         const R c = R(FD<S>::a(l));
         const R rsp = Off<R>::on ? (rc + f(QR, l)) + r2 : rc + f(QR, l);
         const R rsm = Off<R>::on ? (f(QR, -l) + rc) + r2 : f(QR, -l) + rc;
-        const R qp = c * (rsp * (ac + f(QA, l)));
-        const R qm = c * (rsm * (f(QA, -l) + ac));
-        fr += qp - qm;
-        fu += qp * (uc + f(QU, l)) - qm * (f(QU, -l) + uc);
-        fv += qp * (vc + f(QV, l)) - qm * (f(QV, -l) + vc);
-        fw += qp * (wc + f(QW, l)) - qm * (f(QW, -l) + wc);
-        fe += qp * (hc + f(QH, l)) - qm * (f(QH, -l) + hc);
+        const R qp = rsp * (ac + f(QA, l));
+        const R qm = rsm * (f(QA, -l) + ac);
+        fr += c * (qp - qm);
+        fu += c * (qp * (uc + f(QU, l)) - qm * (f(QU, -l) + uc));
+        fv += c * (qp * (vc + f(QV, l)) - qm * (f(QV, -l) + vc));
+        fw += c * (qp * (wc + f(QW, l)) - qm * (f(QW, -l) + wc));
+        fe += c * (qp * (hc + f(QH, l)) - qm * (f(QH, -l) + hc));
     }
     const R sq = R(-0.5) * xs, sc = R(-0.25) * xs;
     const R cr = sq * fr, cu = sc * fu, cv = sc * fv, cw = sc * fw;
