@@ -53,6 +53,7 @@ EXPORTS = [
     "cns_profile_read", "cns_time_steps", "cns_device_bytes",
     "cns_write_fields", "cns_read_fields", "cns_write_grid", "cns_get_control", "cns_set_control",
     "cns_channel_profiles", "cns_friction_reynolds", "cns_flow_stats", "cns_set_pencil", "cns_tune",
+    "cns_step_stats",
     "cns_batch_create", "cns_batch_tune", "cns_batch_step", "cns_batch_launch", "cns_batch_wait", "cns_batch_time_steps", "cns_batch_group_info", "cns_batch_set_shape",
     "cns_batch_groups", "cns_batch_destroy", "cns_batch_last_error",
 ]
@@ -106,6 +107,7 @@ def load(path=None):
     L.cns_flow_stats.argtypes = [vp, dp]
     L.cns_set_pencil.argtypes = [vp, C.c_int]
     L.cns_tune.argtypes = [vp, C.POINTER(C.c_int)]
+    L.cns_step_stats.argtypes = [vp, C.c_int, C.c_int, dp]
     L.cns_batch_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(vp)]
     L.cns_batch_tune.argtypes = [vp]
     L.cns_batch_step.argtypes = [vp, C.c_int]

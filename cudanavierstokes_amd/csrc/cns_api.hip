@@ -150,6 +150,8 @@ struct cns_solver {
     int cur = 0;
     double *d_par1 = nullptr, *d_par2 = nullptr, *d_time = nullptr;
     double *d_prof = nullptr;  // 10*mx wall-normal profiles (allocated on first use)
+    double *d_stats = nullptr;  // flow statistics samples of cns_step_stats
+    int stats_cap = 0;
     int par_cap = 0;
     bool visc_from_state = true;
     bool have_state = false;
@@ -468,6 +470,7 @@ extern "C" int cns_destroy(cns_solver *s) {
     fr(s->d_par2);
     fr(s->d_time);
     fr(s->d_prof);
+    fr(s->d_stats);
     for (auto &e : s->ev) (void)hipEventDestroy(e);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
@@ -1389,6 +1392,53 @@ static int stats_t(cns_solver *s, double *out) {
     out[1] = sum[2] / sum[0];
     out[2] = sum[3] / vol;
     out[3] = sum[0] / vol;
+    return CNS_OK;
+}
+
+// flow statistics of the current state into dst[0..3] on the device
+template <typename R>
+static int queue_stats(cns_solver *s, double *dst) {
+    const R *q, *lo;
+    state_arrays<R>(s, q, lo);
+    switch (s->SV) {
+        case 1: launch_stats<R, 1>(s, q, lo); break;
+        case 2: launch_stats<R, 2>(s, q, lo); break;
+        case 3: launch_stats<R, 3>(s, q, lo); break;
+        default: launch_stats<R, 4>(s, q, lo); break;
+    }
+    LAUNCHED(s, "k_flow_stats");
+    hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(64), 0, s->stream, (const double *)s->d_part,
+                       s->cfg.len_x * s->cfg.len_y * s->cfg.len_z, dst);
+    LAUNCHED(s, "k_stats_final");
+    return CNS_OK;
+}
+
+extern "C" int cns_step_stats(cns_solver *s, int nsteps, int every, double *stats) {
+    if (!s || !stats) return fail(s, CNS_ERR_ARG, "cns_step_stats: null argument");
+    if (nsteps < 0 || every < 1 || nsteps % every != 0)
+        return fail(s, CNS_ERR_ARG, "cns_step_stats: n_steps must be a non-negative multiple of every >= 1");
+    int rc = diag_ready(s, "cns_step_stats");
+    if (rc) return rc;
+    const int nsamp = nsteps / every;
+    if (nsamp == 0) return CNS_OK;
+    if (4 * nsamp > s->stats_cap) {
+        if (s->d_stats) {
+            (void)hipFree(s->d_stats);
+            s->dev_bytes -= (uint64_t)s->stats_cap * sizeof(double);
+            s->d_stats = nullptr;
+            s->stats_cap = 0;
+        }
+        if ((rc = balloc(s, (void **)&s->d_stats, 4 * (size_t)nsamp * sizeof(double)))) return rc;
+        s->stats_cap = 4 * nsamp;
+    }
+    s->bsteps = 0;
+    for (int k = 0; k < nsamp; k++) {
+        if ((rc = queue_steps(s, every, false, false))) return rc;
+        if ((rc = s->fp32 ? queue_stats<float>(s, s->d_stats + 4 * k) : queue_stats<double>(s, s->d_stats + 4 * k)))
+            return rc;
+    }
+    HIPCHK(s, hipMemcpyAsync(stats, s->d_stats, 4 * (size_t)nsamp * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
     return CNS_OK;
 }
 

@@ -199,4 +199,19 @@ __global__ void k_flow_stats(Params P, long fs, const R *q, const R *lo, int wal
         for (int a = 0; a < 4; a++) part[a * RED_BLOCKS + blockIdx.x] = sh[a][0];
 }
 
+// The statistics from the k_flow_stats partials, summed in block order by
+// one thread (the order of the host's sum in cns_flow_stats, so both give
+// the same bits): out[0..3] = kinetic energy, enstrophy, mean (div u)^2,
+// mean density.
+static __global__ void k_stats_final(const double *part, double vol, double *out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double sum[4] = {0, 0, 0, 0};
+    for (int a = 0; a < 4; a++)
+        for (int b = 0; b < RED_BLOCKS; b++) sum[a] += part[a * RED_BLOCKS + b];
+    out[0] = sum[1] / sum[0];
+    out[1] = sum[2] / sum[0];
+    out[2] = sum[3] / vol;
+    out[3] = sum[0] / vol;
+}
+
 }  // namespace cns

@@ -243,6 +243,15 @@ class Solver:
         self._chk(self.L.cns_flow_stats(self.h, out), "cns_flow_stats")
         return dict(kinetic=out[0], enstrophy=out[1], dil2=out[2], rho=out[3])
 
+    def step_stats(self, nsteps, every):
+        """nsteps steps (as nsteps/every cns_step(every) calls) with the flow
+        statistics of the state after each group reduced on the device:
+        array (nsteps/every, 4) of kinetic energy, enstrophy, mean squared
+        dilatation, mean density"""
+        out = np.empty(4 * (nsteps // every))
+        self._chk(self.L.cns_step_stats(self.h, int(nsteps), int(every), out), "cns_step_stats")
+        return out.reshape(-1, 4)
+
     def write_fields(self, directory, timestep):
         """<directory>/{r,u,v,w,e}.<timestep:07d>.bin, float64 [mz][my][mx]
         (writeField, src/init.cpp:24-30)"""

@@ -151,6 +151,13 @@ int cns_channel_profiles(cns_solver *s, double *prof);
 int cns_friction_reynolds(cns_solver *s, double *re_tau);
 int cns_flow_stats(cns_solver *s, double *out);
 
+/* The statistics of cns_flow_stats folded into the step loop: n_steps steps
+ * taken as n_steps/every successive cns_step(every) calls (a CFL check opens
+ * each), each followed by the statistics of the state, reduced on the
+ * device with nothing read back until the end; stats receives 4 values per
+ * sample (n_steps/every samples; n_steps must be a multiple of every). */
+int cns_step_stats(cns_solver *s, int n_steps, int every, double *stats);
+
 /* Field files of writeField / initField (src/init.cpp:13-30), written and
  * read like saveFileMPI / readFileMPI (src/comm.cpp:205-279):
  * <dir>/<c>.<timestep, %07d>.bin for c = r, u, v, w, e, each the whole grid

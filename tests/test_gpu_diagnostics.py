@@ -82,3 +82,31 @@ def test_flow_stats(name, cfg, kind):
     if kind == "tgv":
         # the vortex keeps its kinetic energy close to 1/8 over a few steps
         assert abs(st["kinetic"] - 0.125) < 1e-3
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_step_stats_folded_into_the_step_loop(precision):
+    """cns_step_stats(6, 2) runs the kernels of three cns_step(2) calls, each
+    followed by cns_flow_stats, with the sums on the device: same bits"""
+    cfg = cns.taylor_green(n=24, order=4, scheme="rk4", check_cfl=2, precision=precision)
+    flush_print(f"[case] step stats {cfg}")
+    with cns.Solver(cfg) as a:
+        a.init_taylor_green()
+        got = a.step_stats(6, 2)
+        end_a = a.get_state()
+        with pytest.raises(cns.CnsError):
+            a.step_stats(5, 2)
+    with cns.Solver(cfg) as b:
+        b.init_taylor_green()
+        want = []
+        for _ in range(3):
+            b.step(2, outputs=False)
+            st = b.flow_stats()
+            want.append([st["kinetic"], st["enstrophy"], st["dil2"], st["rho"]])
+        end_b = b.get_state()
+    assert got.shape == (3, 4)
+    assert np.array_equal(got, np.array(want))
+    for x, y in zip(end_a, end_b):
+        assert np.array_equal(x, y)
+    # kinetic energy decays, enstrophy grows at the start of the vortex
+    assert got[2, 0] < got[0, 0] and got[2, 1] > got[0, 1]
