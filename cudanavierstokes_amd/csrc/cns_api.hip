@@ -273,13 +273,13 @@ static void build_grid(cns_solver *s) {
                                    s->cVF[2 * SV - it] * xpp[i] * (xp[i] * xp[i] * xp[i]) * s->d_dx);
 }
 
-// Workgroup grid of the stage kernels: x-y tiles times z chunks of zc
-// planes.  Default chunk length (measured on 256^3 and 128^3 Taylor-Green
-// cases): the FP64 kernel (one workgroup per CU) prefers long marches, about
-// one workgroup per CU, chunks of 8..256 planes; the FP32 kernel (two
-// workgroups per CU) prefers about 4096 workgroups; small grids want chunks
-// of 8.
-// cns_tune measures instead.
+// Workgroup grid of the stage kernels: x-y tiles (tile_x() x 16 columns)
+// times z chunks of zc planes.  Default chunk length (measured on 256^3 and
+// 128^3 Taylor-Green cases): the FP64 kernel (one workgroup per CU) prefers
+// long marches, about one workgroup per CU, chunks of 8..256 planes; the
+// FP32 kernel with mu = T/Re (one 512-lane workgroup per CU) about four
+// rounds of workgroups; small grids want chunks of 8.  cns_tune measures
+// instead.
 static void set_pencil(cns_solver *s, int zc) {
     const cns_config &c = s->cfg;
     s->zc = std::max(1, std::min(zc, c.nz));
@@ -291,11 +291,14 @@ static void set_pencil(cns_solver *s, int zc) {
 
 static void launch_geometry(cns_solver *s) {
     const cns_config &c = s->cfg;
-    s->ntx = (c.nx + TILE_X - 1) / TILE_X;
+    const bool fp32 = c.precision == CNS_FP32, unitv = c.visc_exp == 1.0;
+    const int tx = tile_x_of(fp32, unitv);
+    s->ntx = (c.nx + tx - 1) / tx;
     s->nty = (c.ny + TILE_Y - 1) / TILE_Y;
     const long cols = (long)s->ntx * s->nty;
-    const bool fp32 = c.precision == CNS_FP32;
-    const long target = fp32 ? 4096 : 256;
+    // workgroups to aim at: about four per CU slot for the FP32 kernels with
+    // mu = T/Re (one 512-lane workgroup per CU), one round otherwise
+    const long target = fp32 ? (unitv ? 1024 : 512) : 256;
     long zc = ((long)c.nz * cols + target - 1) / target;
     zc = std::max<long>(zc, 8);
     zc = std::min<long>(zc, fp32 ? 128 : 256);
@@ -1714,7 +1717,7 @@ static int queue_stage(cns_batch *b, const BatchGroup &g, int st) {
     const Job<R> *jobs = static_cast<const Job<R> *>(g.d_jobs) + (size_t)st * nm;
     // grid 0: one workgroup per work item (the dispatcher balances)
     const int grid = g.grid > 0 ? std::max(1, std::min(g.grid, g.nwork)) : std::max(1, g.nwork);
-    BCHK(b, launch_dil_batch<R>(g.SV, g.wallx, g.nonuni, grid, b->stream, jobs, nm, g.nwork, g.remap));
+    BCHK(b, launch_dil_batch<R>(g.SV, g.wallx, g.nonuni, g.unitv, grid, b->stream, jobs, nm, g.nwork, g.remap));
     BLAUNCHED(b, "k_dil_batch");
     // three update operands only in the last Kutta stage
     const bool k3 = g.scheme == CNS_RK3_KUTTA && st == 2;

@@ -14,17 +14,18 @@ namespace cns {
 
 template <typename R, int S, int SV, bool UNITV, bool K3>
 static hipError_t go_rhs_v(bool wallx, bool nonuni, int nb, hipStream_t st, const Params &P, const StagePtrs<R> &A) {
-    const dim3 g(nb), b(TILE_X * TILE_Y);
+    constexpr int TX = tile_x<R, UNITV>();
+    const dim3 g(nb), b(TX * TILE_Y);
     if (wallx) {
         if (nonuni)
-            hipLaunchKernelGGL((k_rhs<R, S, SV, TILE_X, TILE_Y, true, true, UNITV, K3>), g, b, 0, st, P, A);
+            hipLaunchKernelGGL((k_rhs<R, S, SV, TX, TILE_Y, true, true, UNITV, K3>), g, b, 0, st, P, A);
         else
-            hipLaunchKernelGGL((k_rhs<R, S, SV, TILE_X, TILE_Y, true, false, UNITV, K3>), g, b, 0, st, P, A);
+            hipLaunchKernelGGL((k_rhs<R, S, SV, TX, TILE_Y, true, false, UNITV, K3>), g, b, 0, st, P, A);
     } else {
         if (nonuni)
-            hipLaunchKernelGGL((k_rhs<R, S, SV, TILE_X, TILE_Y, false, true, UNITV, K3>), g, b, 0, st, P, A);
+            hipLaunchKernelGGL((k_rhs<R, S, SV, TX, TILE_Y, false, true, UNITV, K3>), g, b, 0, st, P, A);
         else
-            hipLaunchKernelGGL((k_rhs<R, S, SV, TILE_X, TILE_Y, false, false, UNITV, K3>), g, b, 0, st, P, A);
+            hipLaunchKernelGGL((k_rhs<R, S, SV, TX, TILE_Y, false, false, UNITV, K3>), g, b, 0, st, P, A);
     }
     return hipGetLastError();
 }
@@ -40,41 +41,51 @@ static hipError_t go_rhs(bool wallx, bool nonuni, int nb, hipStream_t st, const 
               : go_rhs_v<R, S, SV, false, false>(wallx, nonuni, nb, st, P, A);
 }
 
+template <typename R, int SV, int TX>
+static hipError_t go_dil_t(bool wallx, bool nonuni, int nb, hipStream_t st, const Params &P, const StagePtrs<R> &A,
+                           R *dil) {
+    const dim3 g(nb), b(TX * TILE_Y);
+    if (wallx) {
+        if (nonuni)
+            hipLaunchKernelGGL((k_dil<R, SV, TX, TILE_Y, true, true>), g, b, 0, st, P, A, dil);
+        else
+            hipLaunchKernelGGL((k_dil<R, SV, TX, TILE_Y, true, false>), g, b, 0, st, P, A, dil);
+    } else {
+        if (nonuni)
+            hipLaunchKernelGGL((k_dil<R, SV, TX, TILE_Y, false, true>), g, b, 0, st, P, A, dil);
+        else
+            hipLaunchKernelGGL((k_dil<R, SV, TX, TILE_Y, false, false>), g, b, 0, st, P, A, dil);
+    }
+    return hipGetLastError();
+}
+
+// the dilatation kernel tiles the grid as the RHS kernel of the same solver
 template <typename R, int SV>
 static hipError_t go_dil(bool wallx, bool nonuni, int nb, hipStream_t st, const Params &P, const StagePtrs<R> &A,
                          R *dil) {
-    const dim3 g(nb), b(TILE_X * TILE_Y);
-    if (wallx) {
-        if (nonuni)
-            hipLaunchKernelGGL((k_dil<R, SV, TILE_X, TILE_Y, true, true>), g, b, 0, st, P, A, dil);
-        else
-            hipLaunchKernelGGL((k_dil<R, SV, TILE_X, TILE_Y, true, false>), g, b, 0, st, P, A, dil);
-    } else {
-        if (nonuni)
-            hipLaunchKernelGGL((k_dil<R, SV, TILE_X, TILE_Y, false, true>), g, b, 0, st, P, A, dil);
-        else
-            hipLaunchKernelGGL((k_dil<R, SV, TILE_X, TILE_Y, false, false>), g, b, 0, st, P, A, dil);
-    }
-    return hipGetLastError();
+    if (tile_x_of(sizeof(R) == 4, P.unit_visc != 0) == 32)
+        return go_dil_t<R, SV, tile_x<R, true>()>(wallx, nonuni, nb, st, P, A, dil);
+    return go_dil_t<R, SV, 16>(wallx, nonuni, nb, st, P, A, dil);
 }
 
 template <typename R, int S, int SV, bool UNITV, bool K3>
 static hipError_t go_rhs_batch_v(bool wallx, bool nonuni, int grid, hipStream_t st, const Job<R> *jobs, int njobs,
                                  int nwork, int remap) {
-    const dim3 g(grid), b(TILE_X * TILE_Y);
+    constexpr int TX = tile_x<R, UNITV>();
+    const dim3 g(grid), b(TX * TILE_Y);
     if (wallx) {
         if (nonuni)
-            hipLaunchKernelGGL((k_rhs_batch<R, S, SV, TILE_X, TILE_Y, true, true, UNITV, K3>), g, b, 0, st, jobs, njobs,
+            hipLaunchKernelGGL((k_rhs_batch<R, S, SV, TX, TILE_Y, true, true, UNITV, K3>), g, b, 0, st, jobs, njobs,
                                nwork, remap);
         else
-            hipLaunchKernelGGL((k_rhs_batch<R, S, SV, TILE_X, TILE_Y, true, false, UNITV, K3>), g, b, 0, st, jobs, njobs,
+            hipLaunchKernelGGL((k_rhs_batch<R, S, SV, TX, TILE_Y, true, false, UNITV, K3>), g, b, 0, st, jobs, njobs,
                                nwork, remap);
     } else {
         if (nonuni)
-            hipLaunchKernelGGL((k_rhs_batch<R, S, SV, TILE_X, TILE_Y, false, true, UNITV, K3>), g, b, 0, st, jobs, njobs,
+            hipLaunchKernelGGL((k_rhs_batch<R, S, SV, TX, TILE_Y, false, true, UNITV, K3>), g, b, 0, st, jobs, njobs,
                                nwork, remap);
         else
-            hipLaunchKernelGGL((k_rhs_batch<R, S, SV, TILE_X, TILE_Y, false, false, UNITV, K3>), g, b, 0, st, jobs,
+            hipLaunchKernelGGL((k_rhs_batch<R, S, SV, TX, TILE_Y, false, false, UNITV, K3>), g, b, 0, st, jobs,
                                njobs, nwork, remap);
     }
     return hipGetLastError();
@@ -90,26 +101,34 @@ static hipError_t go_rhs_batch(bool wallx, bool nonuni, bool unitv, bool k3, int
               : go_rhs_batch_v<R, S, SV, false, false>(wallx, nonuni, grid, st, jobs, njobs, nwork, remap);
 }
 
-template <typename R, int SV>
-static hipError_t go_dil_batch(bool wallx, bool nonuni, int grid, hipStream_t st, const Job<R> *jobs, int njobs,
-                               int nwork, int remap) {
-    const dim3 g(grid), b(TILE_X * TILE_Y);
+template <typename R, int SV, int TX>
+static hipError_t go_dil_batch_t(bool wallx, bool nonuni, int grid, hipStream_t st, const Job<R> *jobs, int njobs,
+                                 int nwork, int remap) {
+    const dim3 g(grid), b(TX * TILE_Y);
     if (wallx) {
         if (nonuni)
-            hipLaunchKernelGGL((k_dil_batch<R, SV, TILE_X, TILE_Y, true, true>), g, b, 0, st, jobs, njobs, nwork,
+            hipLaunchKernelGGL((k_dil_batch<R, SV, TX, TILE_Y, true, true>), g, b, 0, st, jobs, njobs, nwork,
                                remap);
         else
-            hipLaunchKernelGGL((k_dil_batch<R, SV, TILE_X, TILE_Y, true, false>), g, b, 0, st, jobs, njobs, nwork,
+            hipLaunchKernelGGL((k_dil_batch<R, SV, TX, TILE_Y, true, false>), g, b, 0, st, jobs, njobs, nwork,
                                remap);
     } else {
         if (nonuni)
-            hipLaunchKernelGGL((k_dil_batch<R, SV, TILE_X, TILE_Y, false, true>), g, b, 0, st, jobs, njobs, nwork,
+            hipLaunchKernelGGL((k_dil_batch<R, SV, TX, TILE_Y, false, true>), g, b, 0, st, jobs, njobs, nwork,
                                remap);
         else
-            hipLaunchKernelGGL((k_dil_batch<R, SV, TILE_X, TILE_Y, false, false>), g, b, 0, st, jobs, njobs, nwork,
+            hipLaunchKernelGGL((k_dil_batch<R, SV, TX, TILE_Y, false, false>), g, b, 0, st, jobs, njobs, nwork,
                                remap);
     }
     return hipGetLastError();
+}
+
+template <typename R, int SV>
+static hipError_t go_dil_batch(bool wallx, bool nonuni, bool unitv, int grid, hipStream_t st, const Job<R> *jobs,
+                               int njobs, int nwork, int remap) {
+    if (tile_x_of(sizeof(R) == 4, unitv) == 32)
+        return go_dil_batch_t<R, SV, tile_x<R, true>()>(wallx, nonuni, grid, st, jobs, njobs, nwork, remap);
+    return go_dil_batch_t<R, SV, 16>(wallx, nonuni, grid, st, jobs, njobs, nwork, remap);
 }
 
 #define CNS_CAT2(a, b) a##b
@@ -168,9 +187,9 @@ hipError_t LAUNCH_RHS_BATCH(int sv, bool wallx, bool nonuni, bool unitv, bool k3
 
 // here CNS_S is the viscous half width of the dilatation kernel
 template <typename R>
-hipError_t LAUNCH_DIL_BATCH(bool wallx, bool nonuni, int grid, hipStream_t st, const Job<R> *jobs, int njobs,
-                            int nwork, int remap) {
-    return go_dil_batch<R, CNS_S>(wallx, nonuni, grid, st, jobs, njobs, nwork, remap);
+hipError_t LAUNCH_DIL_BATCH(bool wallx, bool nonuni, bool unitv, int grid, hipStream_t st, const Job<R> *jobs,
+                            int njobs, int nwork, int remap) {
+    return go_dil_batch<R, CNS_S>(wallx, nonuni, unitv, grid, st, jobs, njobs, nwork, remap);
 }
 
 #define CNS_INSTANTIATE(R)                                                                                        \
@@ -178,7 +197,7 @@ hipError_t LAUNCH_DIL_BATCH(bool wallx, bool nonuni, int grid, hipStream_t st, c
     template hipError_t LAUNCH_DIL<R>(bool, bool, int, hipStream_t, const Params &, const StagePtrs<R> &, R *);  \
     template hipError_t LAUNCH_RHS_BATCH<R>(int, bool, bool, bool, bool, int, hipStream_t, const Job<R> *, int, int, \
                                             int);                                                                 \
-    template hipError_t LAUNCH_DIL_BATCH<R>(bool, bool, int, hipStream_t, const Job<R> *, int, int, int);
+    template hipError_t LAUNCH_DIL_BATCH<R>(bool, bool, bool, int, hipStream_t, const Job<R> *, int, int, int);
 
 #if defined(CNS_R)
 CNS_INSTANTIATE(CNS_R)

@@ -6,8 +6,16 @@
 
 namespace cns {
 
-constexpr int TILE_X = 16;
+// Tiles are TX x 16 columns.  The FP32 kernels with mu = T/Re run two waves
+// per SIMD, so a 512-lane workgroup (32 x 16 tile: 384 halo points for 512
+// interior ones instead of 256 for 256) fits; the others keep one wave per
+// SIMD, which allows 256 lanes (16 x 16).
 constexpr int TILE_Y = 16;
+template <typename R, bool UNITV>
+constexpr int tile_x() {
+    return (sizeof(R) == 4 && UNITV) ? 32 : 16;
+}
+inline int tile_x_of(bool fp32, bool unitv) { return (fp32 && unitv) ? 32 : 16; }
 
 #define CNS_DECLARE_LAUNCH(N)                                                                                  \
     template <typename R>                                                                                      \
@@ -20,8 +28,8 @@ constexpr int TILE_Y = 16;
     hipError_t launch_rhs_batch_s##N(int sv, bool wallx, bool nonuni, bool unitv, bool k3, int grid,           \
                                      hipStream_t st, const Job<R> *jobs, int njobs, int nwork, int remap);     \
     template <typename R>                                                                                      \
-    hipError_t launch_dil_batch_s##N(bool wallx, bool nonuni, int grid, hipStream_t st, const Job<R> *jobs,   \
-                                     int njobs, int nwork, int remap);
+    hipError_t launch_dil_batch_s##N(bool wallx, bool nonuni, bool unitv, int grid, hipStream_t st,           \
+                                     const Job<R> *jobs, int njobs, int nwork, int remap);
 CNS_DECLARE_LAUNCH(1)
 CNS_DECLARE_LAUNCH(2)
 CNS_DECLARE_LAUNCH(3)
@@ -66,13 +74,13 @@ inline hipError_t launch_rhs_batch(int s, int sv, bool wallx, bool nonuni, bool 
 }
 
 template <typename R>
-inline hipError_t launch_dil_batch(int sv, bool wallx, bool nonuni, int grid, hipStream_t st, const Job<R> *jobs,
-                                   int njobs, int nwork, int remap) {
+inline hipError_t launch_dil_batch(int sv, bool wallx, bool nonuni, bool unitv, int grid, hipStream_t st,
+                                   const Job<R> *jobs, int njobs, int nwork, int remap) {
     switch (sv) {
-    case 1: return launch_dil_batch_s1<R>(wallx, nonuni, grid, st, jobs, njobs, nwork, remap);
-    case 2: return launch_dil_batch_s2<R>(wallx, nonuni, grid, st, jobs, njobs, nwork, remap);
-    case 3: return launch_dil_batch_s3<R>(wallx, nonuni, grid, st, jobs, njobs, nwork, remap);
-    case 4: return launch_dil_batch_s4<R>(wallx, nonuni, grid, st, jobs, njobs, nwork, remap);
+    case 1: return launch_dil_batch_s1<R>(wallx, nonuni, unitv, grid, st, jobs, njobs, nwork, remap);
+    case 2: return launch_dil_batch_s2<R>(wallx, nonuni, unitv, grid, st, jobs, njobs, nwork, remap);
+    case 3: return launch_dil_batch_s3<R>(wallx, nonuni, unitv, grid, st, jobs, njobs, nwork, remap);
+    case 4: return launch_dil_batch_s4<R>(wallx, nonuni, unitv, grid, st, jobs, njobs, nwork, remap);
     }
     return hipErrorInvalidValue;
 }
