@@ -276,10 +276,9 @@ static void build_grid(cns_solver *s) {
 // Workgroup grid of the stage kernels: x-y tiles (tile_x() x 16 columns)
 // times z chunks of zc planes.  Default chunk length (measured on 256^3 and
 // 128^3 Taylor-Green cases): the FP64 kernel (one workgroup per CU) prefers
-// long marches, about one workgroup per CU, chunks of 8..256 planes; the
-// FP32 kernel with mu = T/Re (one 512-lane workgroup per CU) about four
-// rounds of workgroups; small grids want chunks of 8.  cns_tune measures
-// instead.
+// long marches, about one workgroup per CU, chunks of 8..256 planes, and so
+// does the FP32 kernel with mu = T/Re (one 512-lane workgroup per CU); small
+// grids want chunks of 8.  cns_tune measures instead.
 static void set_pencil(cns_solver *s, int zc) {
     const cns_config &c = s->cfg;
     s->zc = std::max(1, std::min(zc, c.nz));
@@ -296,12 +295,13 @@ static void launch_geometry(cns_solver *s) {
     s->ntx = (c.nx + tx - 1) / tx;
     s->nty = (c.ny + TILE_Y - 1) / TILE_Y;
     const long cols = (long)s->ntx * s->nty;
-    // workgroups to aim at: about four per CU slot for the FP32 kernels with
-    // mu = T/Re (one 512-lane workgroup per CU), one round otherwise
-    const long target = fp32 ? (unitv ? 1024 : 512) : 256;
+    // workgroups to aim at: one round (one workgroup per CU: 512 lanes at two
+    // waves per SIMD, or 256 at one), twice that for FP32 with a general
+    // viscosity law
+    const long target = (fp32 && !unitv) ? 512 : 256;
     long zc = ((long)c.nz * cols + target - 1) / target;
     zc = std::max<long>(zc, 8);
-    zc = std::min<long>(zc, fp32 ? 128 : 256);
+    zc = std::min<long>(zc, 256);
     // CNS_PENCIL=<planes>: z-chunk length override
     if (const char *pz = std::getenv("CNS_PENCIL")) {
         const int v = std::atoi(pz);
