@@ -321,6 +321,7 @@ static void fill_params(cns_solver *s) {
     P.ntx = s->ntx;
     P.nty = s->nty;
     P.nzc = s->nzc;
+    P.tile_x = tile_x_of(s->fp32, c.visc_exp == 1.0);
     P.forcing = c.forcing_z;
     P.sponge = c.sponge;
     P.fix_y_work = c.fix_y_work;
@@ -866,7 +867,20 @@ static int run_stage(cns_solver *s, int in_set, Params P, StagePtrs<R> A) {
     A.fs = s->n;
     A.dil = as<R>(s->d_dil);
     fill_unused(A);
-    HIPCHK(s, launch_dil<R>(s->SV, wallx, nonuni, s->nblocks, s->stream, P, A, as<R>(s->d_dil)));
+    // the dilatation kernel needs few registers, so it always runs 512-lane
+    // workgroups on 32 x 16 tiles (fewer halo points per point), with its own
+    // tile grid and z chunks: about 1024 workgroups, chunks of 8..the RHS's
+    Params Pd = P;
+    Pd.tile_x = 32;
+    Pd.ntx = (s->cfg.nx + 31) / 32;
+    {
+        const long dcols = (long)Pd.ntx * Pd.nty;
+        const int dz = (int)std::max<long>(8, std::min<long>(s->zc, (long)s->cfg.nz * dcols / 1024));
+        Pd.zc = std::min(dz, s->cfg.nz);
+        Pd.nzc = (s->cfg.nz + Pd.zc - 1) / Pd.zc;
+    }
+    const int dil_blocks = Pd.ntx * Pd.nty * Pd.nzc;
+    HIPCHK(s, launch_dil<R>(s->SV, wallx, nonuni, dil_blocks, s->stream, Pd, A, as<R>(s->d_dil)));
     LAUNCHED(s, "k_dil");
     if (P.want_visc_max) {
         hipLaunchKernelGGL(k_reset_visc, dim3(1), dim3(1), 0, s->stream, s->d_sc);

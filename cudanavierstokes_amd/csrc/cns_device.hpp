@@ -24,6 +24,7 @@ struct Params {
     int forcing, sponge, fix_y_work, unit_visc;
     int want_visc_max;  // accumulate max(mu/dx^2, ...) for the next CFL check
     int xcd_remap;
+    int tile_x;  // x width of the launch's tiles (16 or 32; host dispatch of the dilatation kernel)
     // stage update, per conserved field:
     //   q_out   = s1 + dt (w_ra ra + w_rb rb + w_r rhs)   (when write_q)
     //   acc_out = s2 + dt (v_r rhs)                       (when write_acc)

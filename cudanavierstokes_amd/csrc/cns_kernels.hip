@@ -59,12 +59,11 @@ static hipError_t go_dil_t(bool wallx, bool nonuni, int nb, hipStream_t st, cons
     return hipGetLastError();
 }
 
-// the dilatation kernel tiles the grid as the RHS kernel of the same solver
+// tile width of the launch as the host set it (P.tile_x)
 template <typename R, int SV>
 static hipError_t go_dil(bool wallx, bool nonuni, int nb, hipStream_t st, const Params &P, const StagePtrs<R> &A,
                          R *dil) {
-    if (tile_x_of(sizeof(R) == 4, P.unit_visc != 0) == 32)
-        return go_dil_t<R, SV, tile_x<R, true>()>(wallx, nonuni, nb, st, P, A, dil);
+    if (P.tile_x == 32) return go_dil_t<R, SV, 32>(wallx, nonuni, nb, st, P, A, dil);
     return go_dil_t<R, SV, 16>(wallx, nonuni, nb, st, P, A, dil);
 }
 
