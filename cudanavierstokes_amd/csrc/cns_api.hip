@@ -870,6 +870,8 @@ static int run_stage(cns_solver *s, int in_set, Params P, StagePtrs<R> A) {
     // the dilatation kernel needs few registers, so it always runs 512-lane
     // workgroups on 32 x 16 tiles (fewer halo points per point), with its own
     // tile grid and z chunks: about 1024 workgroups, chunks of 8..the RHS's
+    // (small grids, where that leaves fewer than 512 workgroups, keep the
+    // RHS kernel's tiles)
     Params Pd = P;
     Pd.tile_x = 32;
     Pd.ntx = (s->cfg.nx + 31) / 32;
@@ -879,7 +881,11 @@ static int run_stage(cns_solver *s, int in_set, Params P, StagePtrs<R> A) {
         Pd.zc = std::min(dz, s->cfg.nz);
         Pd.nzc = (s->cfg.nz + Pd.zc - 1) / Pd.zc;
     }
-    const int dil_blocks = Pd.ntx * Pd.nty * Pd.nzc;
+    int dil_blocks = Pd.ntx * Pd.nty * Pd.nzc;
+    if (dil_blocks < 512) {
+        Pd = P;
+        dil_blocks = s->nblocks;
+    }
     HIPCHK(s, launch_dil<R>(s->SV, wallx, nonuni, dil_blocks, s->stream, Pd, A, as<R>(s->d_dil)));
     LAUNCHED(s, "k_dil");
     if (P.want_visc_max) {

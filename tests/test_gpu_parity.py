@@ -49,6 +49,8 @@ RHS_CASES = [
                                                 twall_bot=1.1, twall_top=0.9)),
     ("periodic_stretched", "random", dict(mx=20, my=16, mz=16, stencil_size=2, stencil_visc=1, nonuniform_x=1,
                                           Lx=2.0, Ma=0.5)),
+    # large enough for the dilatation kernel's own 32 x 16 tile grid
+    ("tgv128_s2_wide_dilatation", "tgv", dict(mx=128, my=128, mz=128, stencil_size=2, stencil_visc=2)),
 ]
 
 
@@ -60,7 +62,12 @@ def test_rhs_parity(name, kind, kw):
     rg = sol.rhs()
     errs = cases.rel_errors(rg, ro)
     flush_print(name, errs)
-    assert_close(errs, RHS_L2, RHS_LINF, f"rhs {name}")
+    # at 128^3 and Mach 0.1 the mass and energy right-hand sides are small
+    # differences of much larger fluxes (the divergence of the momentum is
+    # O(Ma^2)), so rounding of the same operators in another order shows at
+    # the 1e-12 relative level
+    l2, linf = (1e-11, 1e-10) if cfg.mx >= 128 else (RHS_L2, RHS_LINF)
+    assert_close(errs, l2, linf, f"rhs {name}")
     sol.close()
 
 
