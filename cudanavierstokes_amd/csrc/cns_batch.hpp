@@ -98,14 +98,18 @@ __device__ __forceinline__ int job_of(cptr<Job<R>> jobs, int njobs, int w) {
 // range of items (neighbouring tiles of one instance share halo planes in
 // that XCD's L2).
 template <typename R, int S, int SV, int TX, int TY, bool WALLX, bool NONUNI, bool UNITV, bool K3>
-__global__ __launch_bounds__(TX *TY, (rhs_waves<R, UNITV>())) void k_rhs_batch(const Job<R> *__restrict__ jobs, int njobs, int nwork,
-                                                      int remap) {
+__global__ __launch_bounds__((rhs_lanes<R, UNITV, TX, TY>()), (rhs_waves<R, UNITV>())) void k_rhs_batch(
+    const Job<R> *__restrict__ jobs, int njobs, int nwork, int remap) {
     const cptr<Job<R>> cj = as_const(jobs);
     for (int w0 = blockIdx.x; w0 < nwork; w0 += gridDim.x) {
         const int w = remap_block(w0, nwork, remap);
         const cptr<Job<R>> J = cj + job_of(cj, njobs, w);
         const Params P = *(const Params *)&J->P;  // loads stay in the constant space
-        rhs_tile<R, S, SV, TX, TY, WALLX, NONUNI, UNITV, K3>(P, stage_arrays(J, step_parity(J)), w - J->first);
+        if constexpr (rhs_split<R, UNITV>())
+            rhs_tile_split<R, S, SV, TX, TY, WALLX, NONUNI, UNITV, K3>(P, stage_arrays(J, step_parity(J)),
+                                                                      w - J->first);
+        else
+            rhs_tile<R, S, SV, TX, TY, WALLX, NONUNI, UNITV, K3>(P, stage_arrays(J, step_parity(J)), w - J->first);
         __syncthreads();  // LDS reuse by the next item
     }
 }

@@ -15,7 +15,7 @@ namespace cns {
 template <typename R, int S, int SV, bool UNITV, bool K3>
 static hipError_t go_rhs_v(bool wallx, bool nonuni, int nb, hipStream_t st, const Params &P, const StagePtrs<R> &A) {
     constexpr int TX = tile_x<R, UNITV>();
-    const dim3 g(nb), b(TX * TILE_Y);
+    const dim3 g(nb), b(rhs_lanes<R, UNITV, TX, TILE_Y>());
     if (wallx) {
         if (nonuni)
             hipLaunchKernelGGL((k_rhs<R, S, SV, TX, TILE_Y, true, true, UNITV, K3>), g, b, 0, st, P, A);
@@ -71,7 +71,7 @@ template <typename R, int S, int SV, bool UNITV, bool K3>
 static hipError_t go_rhs_batch_v(bool wallx, bool nonuni, int grid, hipStream_t st, const Job<R> *jobs, int njobs,
                                  int nwork, int remap) {
     constexpr int TX = tile_x<R, UNITV>();
-    const dim3 g(grid), b(TX * TILE_Y);
+    const dim3 g(grid), b(rhs_lanes<R, UNITV, TX, TILE_Y>());
     if (wallx) {
         if (nonuni)
             hipLaunchKernelGGL((k_rhs_batch<R, S, SV, TX, TILE_Y, true, true, UNITV, K3>), g, b, 0, st, jobs, njobs,

@@ -148,11 +148,74 @@ __device__ __forceinline__ void vel_stencils(const Params &P, Acc f, const doubl
     }
 }
 
-// Everything else of one direction's part of the RHS; returns the five
-// partial right-hand sides in the reference's term order.
-template <typename R, int S, int SV, int DIR, bool NONUNI, bool UNITV, class Acc>
-__device__ __forceinline__ void dir_terms(const Params &P, Acc f, const double *vs, double xpi, const R g[9],
-                                         const R d2v[3], R out[5]) {
+// The part of one direction's RHS that does not involve the velocity
+// gradients (fluxes, pressure gradient, heat-flux and dilatation
+// derivatives), kept for the stress part below.
+template <typename R>
+struct DirA {
+    R uc, vc, wc, muc, dilc, lamc, d1m, d1t, d1d, d2t, d1p, d1l, cr, cu, cv, cw, ce;
+};
+
+// Split-form convective fluxes of one direction (the telescoped pair form):
+// -d(fgh)/dx ~ -(d/4) sum_l a_l [(f_i+f_{i+l})(g_i+g_{i+l})(h_i+h_{i+l}) - (f_{i-l}+f_i)(...)(...)],
+// out = mass, three momentum and energy fluxes; xs is the direction's
+// inverse spacing (with the x metric).
+template <typename R, int S, int DIR, class Acc>
+__device__ __forceinline__ void dir_flux(const Params &P, Acc f, R xs, R out[5]) {
+    const int QA = (DIR == 0) ? QU : (DIR == 1 ? QV : QW);
+    R fr = R(0), fu = R(0), fv = R(0), fw = R(0), fe = R(0);
+    const R rc = f(QR, 0), ac = f(QA, 0), hc = f(QH, 0);
+    const R uc = f(QU, 0), vc = f(QV, 0), wc = f(QW, 0);
+    // on offset storage rho_a + rho_b = rho'_a + rho'_b + 2 rho0, and h is h'
+    // (the h0 part of the energy flux is h0 times the mass flux, added below)
+    const R r2 = R(2.0 * P.rho0);
+#pragma unroll
+    for (int l = S; l >= 1; l--) {
+        const R c = R(FD<S>::a(l));
+        const R rsp = Off<R>::on ? (rc + f(QR, l)) + r2 : rc + f(QR, l);
+        const R rsm = Off<R>::on ? (f(QR, -l) + rc) + r2 : f(QR, -l) + rc;
+        const R qp = rsp * (ac + f(QA, l));
+        const R qm = rsm * (f(QA, -l) + ac);
+        fr += c * (qp - qm);
+        fu += c * (qp * (uc + f(QU, l)) - qm * (f(QU, -l) + uc));
+        fv += c * (qp * (vc + f(QV, l)) - qm * (f(QV, -l) + vc));
+        fw += c * (qp * (wc + f(QW, l)) - qm * (f(QW, -l) + wc));
+        fe += c * (qp * (hc + f(QH, l)) - qm * (f(QH, -l) + hc));
+    }
+    const R sq = R(-0.5) * xs, sc = R(-0.25) * xs;
+    out[0] = sq * fr;
+    out[1] = sc * fu;
+    out[2] = sc * fv;
+    out[3] = sc * fw;
+    out[4] = Off<R>::on ? sc * fe + R(P.h0) * out[0] : sc * fe;
+}
+
+// Inverse spacing of direction DIR at a point (the x metric in FP64 on
+// offset storage).
+template <typename R, int DIR, bool NONUNI>
+__device__ __forceinline__ R dir_scale(const Params &P, double xpi) {
+    const R d = R(DIR == 0 ? P.d_dx : (DIR == 1 ? P.d_dy : P.d_dz));
+    if (DIR == 0 && NONUNI) return Off<R>::on ? R(P.d_dx * xpi) : d * R(xpi);
+    return d;
+}
+
+// The first derivative of one staged quantity along DIR at viscous order, as
+// vel_stencils forms it.
+template <typename R, int SV, int DIR, bool NONUNI, class Acc>
+__device__ __forceinline__ R d1_of(const Params &P, Acc f, int q, double xpi) {
+    R a1 = R(0);
+#pragma unroll
+    for (int k = SV; k >= 1; k--) a1 += R(FD<SV>::a(k)) * (f(q, k) - f(q, -k));
+    if (Off<R>::on && DIR == 0 && NONUNI) return R((double)a1 * (P.d_dx * xpi));
+    a1 = a1 * R(DIR == 0 ? P.d_dx : (DIR == 1 ? P.d_dy : P.d_dz));
+    if (DIR == 0 && NONUNI) a1 = a1 * R(xpi);
+    return a1;
+}
+
+// FLUX = false leaves the convective fluxes out (zero), for a caller that
+// evaluates them separately with dir_flux.
+template <typename R, int S, int SV, int DIR, bool NONUNI, bool UNITV, bool FLUX = true, class Acc>
+__device__ __forceinline__ DirA<R> dir_terms_a(const Params &P, Acc f, const double *vs, double xpi) {
     const R d = R(DIR == 0 ? P.d_dx : (DIR == 1 ? P.d_dy : P.d_dz));
     const R dd = R(DIR == 0 ? P.d_d2x : (DIR == 1 ? P.d_d2y : P.d_d2z));
     const R uc = f(QU, 0), vc = f(QV, 0), wc = f(QW, 0);
@@ -208,31 +271,21 @@ __device__ __forceinline__ void dir_terms(const Params &P, Acc f, const double *
     d1p = d1p * (Rg * xs);
     const R d1l = d1m * R(P.invPrEc);
 
-    // split-form convective fluxes, telescoped pair form:
-    // -d(fgh)/dx ~ -(d/4) sum_l a_l [(f_i+f_{i+l})(g_i+g_{i+l})(h_i+h_{i+l}) - (f_{i-l}+f_i)(...)(...)]
-    const int QA = (DIR == 0) ? QU : (DIR == 1 ? QV : QW);
-    R fr = R(0), fu = R(0), fv = R(0), fw = R(0), fe = R(0);
-    const R rc = f(QR, 0), ac = f(QA, 0), hc = f(QH, 0);
-    // on offset storage rho_a + rho_b = rho'_a + rho'_b + 2 rho0, and h is h'
-    // (the h0 part of the energy flux is h0 times the mass flux, added below)
-    const R r2 = R(2.0 * P.rho0);
-#pragma unroll
-    for (int l = S; l >= 1; l--) {
-        const R c = R(FD<S>::a(l));
-        const R rsp = Off<R>::on ? (rc + f(QR, l)) + r2 : rc + f(QR, l);
-        const R rsm = Off<R>::on ? (f(QR, -l) + rc) + r2 : f(QR, -l) + rc;
-        const R qp = rsp * (ac + f(QA, l));
-        const R qm = rsm * (f(QA, -l) + ac);
-        fr += c * (qp - qm);
-        fu += c * (qp * (uc + f(QU, l)) - qm * (f(QU, -l) + uc));
-        fv += c * (qp * (vc + f(QV, l)) - qm * (f(QV, -l) + vc));
-        fw += c * (qp * (wc + f(QW, l)) - qm * (f(QW, -l) + wc));
-        fe += c * (qp * (hc + f(QH, l)) - qm * (f(QH, -l) + hc));
-    }
-    const R sq = R(-0.5) * xs, sc = R(-0.25) * xs;
-    const R cr = sq * fr, cu = sc * fu, cv = sc * fv, cw = sc * fw;
-    const R ce = Off<R>::on ? sc * fe + R(P.h0) * cr : sc * fe;
+    R fl[5] = {R(0), R(0), R(0), R(0), R(0)};
+    if constexpr (FLUX) dir_flux<R, S, DIR>(P, f, xs, fl);
+    const R cr = fl[0], cu = fl[1], cv = fl[2], cw = fl[3], ce = fl[4];
+    return DirA<R>{uc, vc, wc, muc, dilc, lamc, d1m, d1t, d1d, d2t, d1p, d1l, cr, cu, cv, cw, ce};
+}
 
+// The stress part of one direction's RHS (velocity gradients g, viscous
+// second derivatives d2v) added to the gradient-free part a; the five
+// partial right-hand sides in the reference's term order.
+template <typename R, int DIR>
+__device__ __forceinline__ void dir_terms_b(const Params &P, const DirA<R> &a, const R g[9], const R d2v[3],
+                                            R out[5]) {
+    const R uc = a.uc, vc = a.vc, wc = a.wc, muc = a.muc, dilc = a.dilc, lamc = a.lamc;
+    const R d1m = a.d1m, d1t = a.d1t, d1d = a.d1d, d2t = a.d2t, d1p = a.d1p, d1l = a.d1l;
+    const R cr = a.cr, cu = a.cu, cv = a.cv, cw = a.cw, ce = a.ce;
     // gradient index map: 0 dudx 1 dvdx 2 dwdx 3 dudy 4 dvdy 5 dwdy 6 dudz 7 dvdz 8 dwdz
     R su, sv, sw, e;
     const R twothird = R(2. / 3.);
@@ -287,6 +340,14 @@ __device__ __forceinline__ void dir_terms(const Params &P, Acc f, const double *
     out[4] = e + ce;
 }
 
+// Everything else of one direction's part of the RHS; returns the five
+// partial right-hand sides in the reference's term order.
+template <typename R, int S, int SV, int DIR, bool NONUNI, bool UNITV, class Acc>
+__device__ __forceinline__ void dir_terms(const Params &P, Acc f, const double *vs, double xpi, const R g[9],
+                                         const R d2v[3], R out[5]) {
+    dir_terms_b<R, DIR>(P, dir_terms_a<R, S, SV, DIR, NONUNI, UNITV>(P, f, vs, xpi), g, d2v, out);
+}
+
 __device__ __forceinline__ void block_max_atomic(double v, unsigned long long *out, double *red) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
@@ -308,6 +369,138 @@ __device__ __forceinline__ long long uniform64(long long v) {
     const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
     const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v >> 32));
     return (long long)(((unsigned long long)hi << 32) | lo);
+}
+
+// Operands of one plane's stage update: s1 and a second operand, ra
+// (low-storage RK3, Kutta) or s2 (RK4's accumulator), rb as a third in the
+// last Kutta stage (K3), and on FP32 storage the low parts of the
+// compensated sets.  An array a stage does not use aliases the stage input,
+// so its loads are cache hits; all are loaded unconditionally.
+template <typename R>
+struct Ops {
+    R e1[5], e2[5], e3[5], e1l[5], e2l[5];
+};
+
+template <typename R, bool K3, class Arr>
+__device__ __forceinline__ void load_ops(Arr arr, int fl, long poff, long fs, unsigned coff, Ops<R> &o) {
+    const R *p1 = arr(0, poff), *p2 = arr((K3 || (fl & F_RA)) ? 1 : 3, poff);
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        o.e1[q] = ld_at(p1 + q * fs, coff);
+        o.e2[q] = ld_at(p2 + q * fs, coff);
+    }
+    if constexpr (K3) {
+        const R *p3 = arr(2, poff);
+#pragma unroll
+        for (int q = 0; q < 5; q++) o.e3[q] = ld_at(p3 + q * fs, coff);
+    } else {
+#pragma unroll
+        for (int q = 0; q < 5; q++) o.e3[q] = R(0);
+    }
+    if constexpr (Off<R>::on) {
+        const R *l1 = arr(7, poff), *l2 = arr(8, poff);
+#pragma unroll
+        for (int q = 0; q < 5; q++) {
+            o.e1l[q] = ld_at(l1 + q * fs, coff);
+            o.e2l[q] = K3 ? R(0) : ld_at(l2 + q * fs, coff);
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 5; q++) o.e1l[q] = o.e2l[q] = R(0);
+    }
+}
+
+// The end of one point's stage (plane k, column i, byte offset coff in the
+// plane): sponge (src/sponge.cu:31-41), the viscous CFL maximum from the
+// point's mu, then q_out = s1 + dt (w_ra ra + w_rb rb + w_r rhs),
+// acc_out = s2 + dt v_r rhs and r_out = rhs as the flags select.  With FP32
+// storage the update is evaluated in FP64 on the compensated pair hi + lo and
+// rounded back to hi and lo (F_COMP).
+template <typename R, bool K3, class Arr>
+__device__ __forceinline__ void stage_update(const Params &P, const StagePtrs<R> &A, Arr arr, int fl,
+                                             const double *s_w, long poff, long fs, unsigned coff, int i, int k,
+                                             double dtd, R rhs[5], const Ops<R> &o, double mu, double inv_dx2,
+                                             double &vmax) {
+    const R dt = R(dtd);
+    if (fl & F_SPONGE) {
+        const long i2 = i + (long)P.mx * k, rs = (long)P.mx * P.mz;
+        const R *qc = A.qin + poff;
+        if constexpr (Off<R>::on) {
+            const double sg = P.spx[i] + P.spz[k];
+            const double off[5] = {P.rho0, 0.0, 0.0, 0.0, P.E0};
+#pragma unroll
+            for (int q = 0; q < 5; q++)
+                rhs[q] += R(sg * (P.ref[i2 + q * rs] - ((double)ld_at(qc + q * fs, coff) + off[q])));
+        } else {
+            const R sg = R(P.spx[i] + P.spz[k]);
+#pragma unroll
+            for (int q = 0; q < 5; q++) rhs[q] += sg * (R(P.ref[i2 + q * rs]) - ld_at(qc + q * fs, coff));
+        }
+    }
+    if (fl & F_VISC) {
+        const double v1 = mu * inv_dx2, v2 = mu * P.d_d2y, v3 = mu * P.d_d2z;
+        const double vm = v1 >= (v2 >= v3 ? v2 : v3) ? v1 : (v2 >= v3 ? v2 : v3);
+        vmax = vmax >= vm ? vmax : vm;
+    }
+    if constexpr (Off<R>::on) {
+        // FP64 update of the compensated pair, rounded to hi + lo
+        const bool comp = fl & F_COMP;
+        if (fl & F_Q) {
+            R *const qo = arr(4, poff);
+            const bool wlo = fl & F_QLO;
+            R *const qol = wlo ? arr(9, poff) : qo;
+#pragma unroll
+            for (int q = 0; q < 5; q++) {
+                double inc = 0.0;
+                if (fl & F_RA) inc += s_w[0] * (double)o.e2[q];
+                if (K3 && (fl & F_RB)) inc += s_w[1] * (double)o.e3[q];
+                inc += s_w[2] * (double)rhs[q];
+                const double x = ((double)o.e1[q] + (comp ? (double)o.e1l[q] : 0.0)) + dtd * inc;
+                const R hi = R(x);
+                st_at(qo + q * fs, coff, hi);
+                if (wlo) st_at(qol + q * fs, coff, R(x - (double)hi));
+            }
+        }
+        if (fl & F_R) {
+            R *const ro = arr(5, poff);
+#pragma unroll
+            for (int q = 0; q < 5; q++) st_at(ro + q * fs, coff, rhs[q]);
+        }
+        if (fl & F_ACC) {
+            R *const ao = arr(6, poff);
+            R *const aol = comp ? arr(10, poff) : ao;
+#pragma unroll
+            for (int q = 0; q < 5; q++) {
+                const double x =
+                    ((double)o.e2[q] + (comp ? (double)o.e2l[q] : 0.0)) + dtd * (s_w[3] * (double)rhs[q]);
+                const R hi = R(x);
+                st_at(ao + q * fs, coff, hi);
+                if (comp) st_at(aol + q * fs, coff, R(x - (double)hi));
+            }
+        }
+    } else {
+        if (fl & F_Q) {
+            R *const qo = arr(4, poff);
+#pragma unroll
+            for (int q = 0; q < 5; q++) {
+                R inc = R(0);
+                if (fl & F_RA) inc += R(s_w[0]) * o.e2[q];
+                if (K3 && (fl & F_RB)) inc += R(s_w[1]) * o.e3[q];
+                inc += R(s_w[2]) * rhs[q];
+                st_at(qo + q * fs, coff, o.e1[q] + dt * inc);
+            }
+        }
+        if (fl & F_R) {
+            R *const ro = arr(5, poff);
+#pragma unroll
+            for (int q = 0; q < 5; q++) st_at(ro + q * fs, coff, rhs[q]);
+        }
+        if (fl & F_ACC) {
+            R *const ao = arr(6, poff);
+#pragma unroll
+            for (int q = 0; q < 5; q++) st_at(ao + q * fs, coff, o.e2[q] + dt * (R(s_w[3]) * rhs[q]));
+        }
+    }
 }
 
 // Fused RHS + stage update.  A TX x TY workgroup owns a tile of x-y columns
@@ -444,7 +637,6 @@ __device__ __forceinline__ void rhs_tile(const Params &P, const StagePtrs<R> &A,
     load_raw(A, (long)z0 * mxy, hoff, hr);
     double vmax = 0.0;
     const double dtd = (P.write_q || P.write_acc) ? *P.dt : 0.0;
-    const R dt = R(dtd);
     const R dpdz = P.forcing ? R(*P.dpdz) : R(0);
     __syncthreads();  // update parameters in LDS
     const int fl = __builtin_amdgcn_readfirstlane(s_flags);
@@ -452,41 +644,11 @@ __device__ __forceinline__ void rhs_tile(const Params &P, const StagePtrs<R> &A,
     for (int k = z0; k < z1; ++k) {
         const int b = (k - z0) & 1;
         const long poff = (long)k * mxy;
-        // operands of this plane's update, always loaded: s1 and a second
-        // operand, ra (low-storage RK3, Kutta) or s2 (RK4's accumulator), and
-        // with K3 (the last Kutta stage) rb as a third; an array a stage does
-        // not use aliases the stage input, so its loads are cache hits.
-        // Issued before the window feed and halo prefetches with FP32
-        // storage, after them (shorter register lifetimes) with FP64, which
-        // measured faster each way.
-        R e1[5], e2[5], e3[5], e1l[5], e2l[5];
-        auto load_update_operands = [&]() {
-            const R *p1 = arr(0, poff), *p2 = arr((K3 || (fl & F_RA)) ? 1 : 3, poff);
-#pragma unroll
-            for (int q = 0; q < 5; q++) {
-                e1[q] = ld_at(p1 + q * fs, coff);
-                e2[q] = ld_at(p2 + q * fs, coff);
-            }
-            if constexpr (K3) {
-                const R *p3 = arr(2, poff);
-#pragma unroll
-                for (int q = 0; q < 5; q++) e3[q] = ld_at(p3 + q * fs, coff);
-            } else {
-#pragma unroll
-                for (int q = 0; q < 5; q++) e3[q] = R(0);
-            }
-            if constexpr (Off<R>::on) {
-                const R *l1 = arr(7, poff), *l2 = arr(8, poff);
-#pragma unroll
-                for (int q = 0; q < 5; q++) {
-                    e1l[q] = ld_at(l1 + q * fs, coff);
-                    e2l[q] = K3 ? R(0) : ld_at(l2 + q * fs, coff);
-                }
-            } else {
-#pragma unroll
-                for (int q = 0; q < 5; q++) e1l[q] = e2l[q] = R(0);
-            }
-        };
+        // Operands of this plane's update, issued before the window feed and
+        // halo prefetches with FP32 storage, after them (shorter register
+        // lifetimes) with FP64, which measured faster each way.
+        Ops<R> ops;
+        auto load_update_operands = [&]() { load_ops<R, K3>(arr, fl, poff, fs, coff, ops); };
         constexpr bool early_ops = sizeof(R) == 4;
         if constexpr (early_ops) load_update_operands();
         __builtin_amdgcn_sched_barrier(0);
@@ -545,86 +707,8 @@ __device__ __forceinline__ void rhs_tile(const Params &P, const StagePtrs<R> &A,
             rhs[3] = rhs[3] + (part[3] + dpdz);
             rhs[4] = rhs[4] + (part[4] + dpdz * wc);
 
-            if (fl & F_SPONGE) {
-                const long i2 = i + (long)P.mx * k, rs = (long)P.mx * P.mz;
-                const R *qc = A.qin + poff;
-                if constexpr (Off<R>::on) {
-                    const double sg = P.spx[i] + P.spz[k];
-                    const double off[5] = {P.rho0, 0.0, 0.0, 0.0, P.E0};
-#pragma unroll
-                    for (int q = 0; q < 5; q++)
-                        rhs[q] += R(sg * (P.ref[i2 + q * rs] - ((double)ld_at(qc + q * fs, coff) + off[q])));
-                } else {
-                    const R sg = R(P.spx[i] + P.spz[k]);
-#pragma unroll
-                    for (int q = 0; q < 5; q++) rhs[q] += sg * (R(P.ref[i2 + q * rs]) - ld_at(qc + q * fs, coff));
-                }
-            }
-            if (fl & F_VISC) {
-                const double mu = (double)FZ(QM, 0);
-                const double v1 = mu * inv_dx2, v2 = mu * P.d_d2y, v3 = mu * P.d_d2z;
-                const double vm = v1 >= (v2 >= v3 ? v2 : v3) ? v1 : (v2 >= v3 ? v2 : v3);
-                vmax = vmax >= vm ? vmax : vm;
-            }
-            if constexpr (Off<R>::on) {
-                // FP64 update of the compensated pair, rounded to hi + lo
-                const bool comp = fl & F_COMP;
-                if (fl & F_Q) {
-                    R *const qo = arr(4, poff);
-                    const bool wlo = fl & F_QLO;
-                    R *const qol = wlo ? arr(9, poff) : qo;
-#pragma unroll
-                    for (int q = 0; q < 5; q++) {
-                        double inc = 0.0;
-                        if (fl & F_RA) inc += s_w[0] * (double)e2[q];
-                        if (K3 && (fl & F_RB)) inc += s_w[1] * (double)e3[q];
-                        inc += s_w[2] * (double)rhs[q];
-                        const double x = ((double)e1[q] + (comp ? (double)e1l[q] : 0.0)) + dtd * inc;
-                        const R hi = R(x);
-                        st_at(qo + q * fs, coff, hi);
-                        if (wlo) st_at(qol + q * fs, coff, R(x - (double)hi));
-                    }
-                }
-                if (fl & F_R) {
-                    R *const ro = arr(5, poff);
-#pragma unroll
-                    for (int q = 0; q < 5; q++) st_at(ro + q * fs, coff, rhs[q]);
-                }
-                if (fl & F_ACC) {
-                    R *const ao = arr(6, poff);
-                    R *const aol = comp ? arr(10, poff) : ao;
-#pragma unroll
-                    for (int q = 0; q < 5; q++) {
-                        const double x =
-                            ((double)e2[q] + (comp ? (double)e2l[q] : 0.0)) + dtd * (s_w[3] * (double)rhs[q]);
-                        const R hi = R(x);
-                        st_at(ao + q * fs, coff, hi);
-                        if (comp) st_at(aol + q * fs, coff, R(x - (double)hi));
-                    }
-                }
-            } else {
-                if (fl & F_Q) {
-                    R *const qo = arr(4, poff);
-#pragma unroll
-                    for (int q = 0; q < 5; q++) {
-                        R inc = R(0);
-                        if (fl & F_RA) inc += R(s_w[0]) * e2[q];
-                        if (K3 && (fl & F_RB)) inc += R(s_w[1]) * e3[q];
-                        inc += R(s_w[2]) * rhs[q];
-                        st_at(qo + q * fs, coff, e1[q] + dt * inc);
-                    }
-                }
-                if (fl & F_R) {
-                    R *const ro = arr(5, poff);
-#pragma unroll
-                    for (int q = 0; q < 5; q++) st_at(ro + q * fs, coff, rhs[q]);
-                }
-                if (fl & F_ACC) {
-                    R *const ao = arr(6, poff);
-#pragma unroll
-                    for (int q = 0; q < 5; q++) st_at(ao + q * fs, coff, e2[q] + dt * (R(s_w[3]) * rhs[q]));
-                }
-            }
+            stage_update<R, K3>(P, A, arr, fl, s_w, poff, fs, coff, i, k, dtd, rhs, ops, (double)FZ(QM, 0), inv_dx2,
+                                vmax);
         }
 #pragma unroll
         for (int q = 0; q < NL; q++)
@@ -634,18 +718,318 @@ __device__ __forceinline__ void rhs_tile(const Params &P, const StagePtrs<R> &A,
     if (P.want_visc_max) block_max_atomic(vmax, P.visc_max_out, red);
 }
 
-// Waves per SIMD the RHS kernel is compiled for: FP32 storage with mu = T/Re
-// fits two (at most 256 VGPRs, with a few scratch spills, which cost less
-// than the latency the second wave hides); the other variants need more
-// registers than two waves can have and keep one.
+// Role-split form of rhs_tile for 2 TX TY lanes, one barrier per plane.  The
+// waves of the first half (plane role) stage the x/y halo of each plane,
+// evaluate the x part (and the y part, or its convective fluxes) from LDS
+// and apply the stage update; the waves of the second half (column role)
+// keep the z register window of the tile's columns, stage the interior of
+// each plane and evaluate the z part (and the y stresses).  Each role holds
+// about half the registers of rhs_tile, so two waves share a SIMD and hide
+// each other's latency.  Per plane k (b = parity of k):
+//   column: push plane k+S into the window, stage plane k's interior,
+//           du/dz and dv/dz of plane k -> LDS;
+//   plane:  stage the halo point of plane k;
+//   -- barrier --
+//   plane:  update of plane k-1 (its z part, and y stresses, came through
+//           LDS), x part and y part or y fluxes of plane k;
+//   column: z part (and y stresses) of plane k -> LDS, window shift.
+// The y stresses go to the column role with a general viscosity law, where
+// that balances the two roles best (measured); with mu = T/Re the plane role
+// evaluates the whole y part.  Each role takes the velocity derivatives it
+// needs from LDS itself, so nothing else crosses between them.  Exchange
+// buffers and the staged planes are double buffered by plane parity.  Both
+// roles run the same barriers; the role is uniform within a wave.  The sums
+// follow rhs_tile: (x part + y part) + z part, as src/cuda_rhs.cu
+// accumulates them.
+template <typename R, int S, int SV, int TX, int TY, bool WALLX, bool NONUNI, bool UNITV, bool K3>
+__device__ __forceinline__ void rhs_tile_split(const Params &P, const StagePtrs<R> &A, const int bb) {
+    constexpr int H = S, LX = TX + 2 * H, LY = TY + 2 * H, NT = TX * TY, W = 2 * H + 1;
+    constexpr int NL = UNITV ? NQ - 1 : NQ;  // staged quantities (mu rebuilt from T when mu = T/Re)
+    constexpr int NH = 2 * H * TY + 2 * H * TX;
+    static_assert(NH <= NT, "one halo point per plane-role lane");
+    // who evaluates the y part: with a general viscosity law the column role
+    // takes its stresses (the plane role keeps the fluxes), which balances
+    // the roles; with mu = T/Re the plane role evaluates all of it
+    constexpr bool YS = !UNITV;
+    static_assert(NT % 64 == 0, "each role is made of whole waves");
+    constexpr unsigned ESZ = sizeof(R);
+    __shared__ R sq[2][NL][LY * LX];
+    // column -> plane role: du/dz, dv/dz of plane k (used after the barrier
+    // of plane k) and, from plane k, the y stresses (4) and the z part (5)
+    // (used after the barrier of plane k+1)
+    __shared__ R xgz[2][2][NT];
+    __shared__ R xyz[2][9][NT];
+    __shared__ double red[2 * NT / 64 + 1];
+    __shared__ double s_w[4];
+    __shared__ long long s_off[11];
+    __shared__ int s_flags;
+    if (threadIdx.x == 0) {
+        s_w[0] = P.w_ra;
+        s_w[1] = P.w_rb;
+        s_w[2] = P.w_r;
+        s_w[3] = P.v_r;
+        const char *q0 = reinterpret_cast<const char *>(A.qin);
+        s_off[0] = reinterpret_cast<const char *>(A.s1) - q0;
+        s_off[1] = reinterpret_cast<const char *>(A.ra) - q0;
+        s_off[2] = reinterpret_cast<const char *>(A.rb) - q0;
+        s_off[3] = reinterpret_cast<const char *>(A.s2) - q0;
+        s_off[4] = reinterpret_cast<const char *>(A.qout) - q0;
+        s_off[5] = reinterpret_cast<const char *>(A.rout) - q0;
+        s_off[6] = reinterpret_cast<const char *>(A.accout) - q0;
+        s_off[7] = reinterpret_cast<const char *>(A.s1lo) - q0;
+        s_off[8] = reinterpret_cast<const char *>(A.s2lo) - q0;
+        s_off[9] = reinterpret_cast<const char *>(A.qoutlo) - q0;
+        s_off[10] = reinterpret_cast<const char *>(A.accoutlo) - q0;
+        s_flags = (P.write_q ? F_Q : 0) | (P.use_ra ? F_RA : 0) | (P.use_rb ? F_RB : 0) |
+                  (P.write_acc ? F_ACC : 0) | (P.write_r ? F_R : 0) | (P.sponge ? F_SPONGE : 0) |
+                  (P.want_visc_max ? F_VISC : 0) | (Off<R>::on && P.comp ? F_COMP : 0) |
+                  (Off<R>::on && P.comp && P.write_qlo ? F_QLO : 0);
+    }
+    auto arr = [&](int which, long poff) -> R * {
+        const long long o = uniform64(s_off[which]);
+        return reinterpret_cast<R *>(const_cast<char *>(reinterpret_cast<const char *>(A.qin)) + o) + poff;
+    };
+
+    const int tile_x = bb % P.ntx;
+    const int rest = bb / P.ntx;
+    const int tile_y = rest % P.nty;
+    const int chunk = rest / P.nty;
+    const int x0 = tile_x * TX, y0 = tile_y * TY;
+    const int z0 = chunk * P.zc;
+    const int z1 = min(z0 + P.zc, P.mz);
+    // the role is uniform per wave; reading it from the first lane lets the
+    // compiler branch on a scalar
+    const bool colrole = __builtin_amdgcn_readfirstlane((int)threadIdx.x) >= NT;
+    const int tid = colrole ? (int)threadIdx.x - NT : (int)threadIdx.x;  // column of the tile
+    const int tx = tid % TX, ty = tid / TX;
+    const int i = x0 + tx, j = y0 + ty;
+    const bool act = (i < P.mx) && (j < P.my);
+    const int cc = (ty + H) * LX + (tx + H);
+    const long mxy = (long)P.mx * P.my;
+    const long fs = A.fs;
+    int cside;
+    const unsigned coff = src_offset<WALLX>(P, i, j, ESZ, cside);
+    const R invRe = R(P.invRe);
+    const R T0 = Off<R>::on ? R(P.T0) : R(0);  // T = T0 + staged T on offset storage
+    double xpi = 1.0;
+    if (NONUNI && act) xpi = P.xp[i];
+    double vmax = 0.0;
+    __syncthreads();  // update parameters in LDS
+    const int fl = __builtin_amdgcn_readfirstlane(s_flags);
+    // staged-plane accessors along x and y (plane parity bk)
+    auto FXb = [&](int bk) {
+        return [&, bk](int q, int a) -> R {
+            return (UNITV && q == QM) ? (Off<R>::on ? T0 + sq[bk][QT][cc + a] : sq[bk][QT][cc + a]) * invRe
+                                      : sq[bk][q][cc + a];
+        };
+    };
+    auto FYb = [&](int bk) {
+        return [&, bk](int q, int a) -> R {
+            return (UNITV && q == QM) ? (Off<R>::on ? T0 + sq[bk][QT][cc + a * LX] : sq[bk][QT][cc + a * LX]) * invRe
+                                      : sq[bk][q][cc + a * LX];
+        };
+    };
+
+    if (colrole) {
+        R win[NL][W];
+#pragma unroll
+        for (int q = 0; q < NL; q++)
+#pragma unroll
+            for (int a = 0; a < W; a++) win[q][a] = R(0);
+        auto push = [&](const R *t, int a) {
+#pragma unroll
+            for (int q = 0; q < NL; q++) win[q][a] = t[q];
+        };
+        // unconditional loads (see rhs_tile)
+#pragma unroll
+        for (int a = 0; a < W - 1; a++) {
+            Raw<R> x;
+            load_raw(A, (long)wrapi(z0 - H + a, P.mz) * mxy, coff, x);
+            R t[NQ];
+            to_prim<R, WALLX, UNITV>(P, x, cside, t);
+            push(t, a);
+        }
+        Raw<R> zr;  // prefetched window feed (plane k + H)
+        load_raw(A, (long)wrapi(z0 + H, P.mz) * mxy, coff, zr);
+        auto FZ = [&](int q, int a) -> R {
+            return (UNITV && q == QM) ? (Off<R>::on ? T0 + win[QT][H + a] : win[QT][H + a]) * invRe : win[q][H + a];
+        };
+        for (int k = z0; k < z1; ++k) {
+            const int b = (k - z0) & 1;
+            R d1z[3], d2z[3];
+            {
+                R t[NQ];
+                to_prim<R, WALLX, UNITV>(P, zr, cside, t);
+                push(t, W - 1);
+#pragma unroll
+                for (int q = 0; q < NL; q++) sq[b][q][cc] = win[q][H];
+                load_raw(A, (long)wrapi(k + 1 + H, P.mz) * mxy, coff, zr);
+            }
+            vel_stencils<R, SV, 2, false>(P, FZ, nullptr, 1.0, d1z, d2z);
+            xgz[b][0][tid] = d1z[0];
+            xgz[b][1][tid] = d1z[1];
+            __syncthreads();  // plane k staged; du/dz, dv/dz of plane k
+            if (act) {
+                const auto FX = FXb(b);
+                const auto FY = FYb(b);
+                R part[5];
+                if constexpr (YS) {
+                    R d1y[3], d2y[3];
+                    vel_stencils<R, SV, 1, false>(P, FY, nullptr, 1.0, d1y, d2y);
+                    const R g[9] = {R(0),   d1_of<R, SV, 0, NONUNI>(P, FX, QV, xpi),
+                                    d1_of<R, SV, 0, NONUNI>(P, FX, QW, xpi),
+                                    d1y[0], d1y[1], d1y[2], d1z[0], d1z[1], d1z[2]};
+                    dir_terms<R, S, SV, 2, false, UNITV>(P, FZ, nullptr, 1.0, g, d2z, part);
+#pragma unroll
+                    for (int q = 0; q < 5; q++) xyz[b][4 + q][tid] = part[q];
+                    dir_terms_b<R, 1>(P, dir_terms_a<R, S, SV, 1, false, UNITV, false>(P, FY, nullptr, 1.0), g, d2y,
+                                      part);
+#pragma unroll
+                    for (int q = 1; q < 5; q++) xyz[b][q - 1][tid] = part[q];
+                } else {
+                    const R g[9] = {R(0), R(0), d1_of<R, SV, 0, NONUNI>(P, FX, QW, xpi), R(0), R(0),
+                                    d1_of<R, SV, 1, false>(P, FY, QW, xpi), d1z[0], d1z[1], d1z[2]};
+                    dir_terms<R, S, SV, 2, false, UNITV>(P, FZ, nullptr, 1.0, g, d2z, part);
+#pragma unroll
+                    for (int q = 0; q < 5; q++) xyz[b][4 + q][tid] = part[q];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < NL; q++)
+#pragma unroll
+                for (int a = 0; a < W - 1; a++) win[q][a] = win[q][a + 1];
+        }
+        __syncthreads();  // y stresses and z part of the last plane
+    } else {
+        // this lane's halo point: x-halo columns first, then y-halo rows
+        const bool hact = tid < NH;
+        int hlx, hly;
+        if (tid < 2 * H * TY) {
+            hly = H + tid / (2 * H);
+            const int c2 = tid % (2 * H);
+            hlx = c2 < H ? c2 : TX + c2;
+        } else {
+            const int s2 = hact ? tid - 2 * H * TY : 0;
+            hlx = H + s2 % TX;
+            const int rr = s2 / TX;
+            hly = rr < H ? rr : TY + rr;
+        }
+        int hside;
+        const unsigned hoff = src_offset<WALLX>(P, x0 + hlx - H, y0 + hly - H, ESZ, hside);
+        const int hslot = hly * LX + hlx;
+        double inv_dx2 = 1.0;
+        double vs[2 * SV + 1];
+#pragma unroll
+        for (int it = 0; it < 2 * SV + 1; it++) vs[it] = 0.0;
+        if (act) {
+            const double dxv = P.dxv[i];
+            inv_dx2 = 1.0 / (dxv * dxv);
+            if (NONUNI) {
+#pragma unroll
+                for (int it = 0; it < 2 * SV + 1; it++) vs[it] = P.vsx[it * P.mx + i];
+            }
+        }
+        const double dtd = (P.write_q || P.write_acc) ? *P.dt : 0.0;
+        const R dpdz = P.forcing ? R(*P.dpdz) : R(0);
+        Raw<R> hr;  // prefetched halo point (plane k)
+        load_raw(A, (long)z0 * mxy, hoff, hr);
+        // the previous plane: x part, y fluxes, update operands, centre w and mu
+        R rx[5], fy[5], wc = R(0);
+        double muc = 0.0;
+        Ops<R> ops;
+#pragma unroll
+        for (int q = 0; q < 5; q++) rx[q] = fy[q] = R(0);
+        // the update of plane kk (parity bk) from its x part, its y part
+        // (fluxes here, stresses in LDS) and its z part in LDS
+        auto finish = [&](int kk, int bk) {
+            R rhs[5];
+            if constexpr (YS) {
+                rhs[0] = rx[0] + fy[0];
+#pragma unroll
+                for (int q = 1; q < 5; q++) rhs[q] = rx[q] + (fy[q] + xyz[bk][q - 1][tid]);
+            } else {
+#pragma unroll
+                for (int q = 0; q < 5; q++) rhs[q] = rx[q] + fy[q];
+            }
+            rhs[0] = rhs[0] + xyz[bk][4][tid];
+            rhs[1] = rhs[1] + xyz[bk][5][tid];
+            rhs[2] = rhs[2] + xyz[bk][6][tid];
+            rhs[3] = rhs[3] + (xyz[bk][7][tid] + dpdz);
+            rhs[4] = rhs[4] + (xyz[bk][8][tid] + dpdz * wc);
+            stage_update<R, K3>(P, A, arr, fl, s_w, (long)kk * mxy, fs, coff, i, kk, dtd, rhs, ops, muc, inv_dx2,
+                                vmax);
+        };
+        for (int k = z0; k < z1; ++k) {
+            const int b = (k - z0) & 1;
+            {
+                R t[NQ];
+                to_prim<R, WALLX, UNITV>(P, hr, hside, t);
+                if (hact) {
+#pragma unroll
+                    for (int q = 0; q < NL; q++) sq[b][q][hslot] = t[q];
+                }
+                load_raw(A, (long)wrapi(k + 1, P.mz) * mxy, hoff, hr);
+            }
+            __syncthreads();  // plane k staged; du/dz, dv/dz of plane k; plane k-1's y stresses and z part
+            if (act) {
+                if (k > z0) finish(k - 1, b ^ 1);
+                const auto FX = FXb(b);
+                const auto FY = FYb(b);
+                R d1x[3], d2x[3];
+                vel_stencils<R, SV, 0, NONUNI>(P, FX, vs, xpi, d1x, d2x);
+                if constexpr (YS) {
+                    const R g[9] = {d1x[0], d1x[1], d1x[2], d1_of<R, SV, 1, false>(P, FY, QU, xpi), R(0), R(0),
+                                    xgz[b][0][tid], xgz[b][1][tid], R(0)};
+                    dir_terms<R, S, SV, 0, NONUNI, UNITV>(P, FX, vs, xpi, g, d2x, rx);
+                    dir_flux<R, S, 1>(P, FY, R(P.d_dy), fy);
+                } else {
+                    R d1y[3], d2y[3];
+                    vel_stencils<R, SV, 1, false>(P, FY, vs, xpi, d1y, d2y);
+                    const R g[9] = {d1x[0], d1x[1], d1x[2], d1y[0], d1y[1], d1y[2],
+                                    xgz[b][0][tid], xgz[b][1][tid], R(0)};
+                    dir_terms<R, S, SV, 0, NONUNI, UNITV>(P, FX, vs, xpi, g, d2x, rx);
+                    dir_terms<R, S, SV, 1, false, UNITV>(P, FY, vs, xpi, g, d2y, fy);
+                }
+                wc = FX(QW, 0);
+                muc = (double)FX(QM, 0);
+                load_ops<R, K3>(arr, fl, (long)k * mxy, fs, coff, ops);
+            }
+        }
+        __syncthreads();  // y stresses and z part of the last plane
+        if (act && z1 > z0) finish(z1 - 1, (z1 - 1 - z0) & 1);
+    }
+    if (P.want_visc_max) block_max_atomic(vmax, P.visc_max_out, red);
+}
+
+// Kernel form per storage type: FP64 runs the role-split tile (two waves
+// per SIMD, each role within 256 registers); FP32 runs rhs_tile.
+template <typename R, bool UNITV>
+constexpr bool rhs_split() {
+    return sizeof(R) == 8;
+}
+// Waves per SIMD the RHS kernel is compiled for: the role-split kernel and
+// FP32 storage with mu = T/Re fit two (at most 256 VGPRs; the latter with a
+// few scratch spills, which cost less than the latency the second wave
+// hides); FP32 with a general viscosity law needs more registers than two
+// waves can have and keeps one.
 template <typename R, bool UNITV>
 constexpr int rhs_waves() {
-    return (sizeof(R) == 4 && UNITV) ? 2 : 1;
+    return (rhs_split<R, UNITV>() || (sizeof(R) == 4 && UNITV)) ? 2 : 1;
+}
+// Lanes of one RHS workgroup for a TX x TY tile.
+template <typename R, bool UNITV, int TX, int TY>
+constexpr int rhs_lanes() {
+    return (rhs_split<R, UNITV>() ? 2 : 1) * TX * TY;
 }
 
 template <typename R, int S, int SV, int TX, int TY, bool WALLX, bool NONUNI, bool UNITV, bool K3>
-__global__ __launch_bounds__(TX *TY, (rhs_waves<R, UNITV>())) void k_rhs(Params P, StagePtrs<R> A) {
-    rhs_tile<R, S, SV, TX, TY, WALLX, NONUNI, UNITV, K3>(P, A, remap_block(blockIdx.x, gridDim.x, P.xcd_remap));
+__global__ __launch_bounds__((rhs_lanes<R, UNITV, TX, TY>()), (rhs_waves<R, UNITV>())) void k_rhs(Params P,
+                                                                                                   StagePtrs<R> A) {
+    const int bb = remap_block(blockIdx.x, gridDim.x, P.xcd_remap);
+    if constexpr (rhs_split<R, UNITV>())
+        rhs_tile_split<R, S, SV, TX, TY, WALLX, NONUNI, UNITV, K3>(P, A, bb);
+    else
+        rhs_tile<R, S, SV, TX, TY, WALLX, NONUNI, UNITV, K3>(P, A, bb);
 }
 
 // Dilatation of the stage input: du/dx + dv/dy + dw/dz at viscous order
