@@ -295,9 +295,9 @@ static void launch_geometry(cns_solver *s) {
     s->ntx = (c.nx + tx - 1) / tx;
     s->nty = (c.ny + TILE_Y - 1) / TILE_Y;
     const long cols = (long)s->ntx * s->nty;
-    // workgroups to aim at: one round (one workgroup per CU: 512 lanes at two
-    // waves per SIMD, or 256 at one), twice that for FP32 with a general
-    // viscosity law
+    // workgroups to aim at: one round, i.e. one workgroup per CU (512 lanes
+    // at two waves per SIMD), or two for FP32 with a general viscosity law
+    // (256 lanes at two waves per SIMD)
     const long target = (fp32 && !unitv) ? 512 : 256;
     long zc = ((long)c.nz * cols + target - 1) / target;
     zc = std::max<long>(zc, 8);
@@ -701,6 +701,9 @@ static void set_offsets(cns_solver *s, double rho0, double E0) {
     const double p0 = (g - 1.0) * E0;
     P.T0 = p0 / (rho0 * s->Rgas);
     P.h0 = (E0 + p0) / rho0;
+    P.dtw_bot = s->cfg.twall_bot - P.T0;
+    P.dtw_top = s->cfg.twall_top - P.T0;
+    P.ch0 = P.cpR * P.T0 - P.h0;
 }
 
 extern "C" int cns_set_state(cns_solver *s, const double *r, const double *u, const double *v, const double *w,
@@ -1137,10 +1140,13 @@ extern "C" int cns_step(cns_solver *s, int nsteps, double *kin, double *enst, do
         LAUNCHED(s, "k_fill");
     }
     if ((rc = queue_steps(s, nsteps, want_par, time != nullptr))) return rc;
+    // kin, enst and time may be host arrays or device arrays (the reference
+    // passes device arrays, src/main.h:35): the runtime takes the direction
+    // from the addresses
     const size_t b = nsteps * sizeof(double);
-    if (kin) HIPCHK(s, hipMemcpyAsync(kin, s->d_par1, b, hipMemcpyDeviceToHost, s->stream));
-    if (enst) HIPCHK(s, hipMemcpyAsync(enst, s->d_par2, b, hipMemcpyDeviceToHost, s->stream));
-    if (time) HIPCHK(s, hipMemcpyAsync(time, s->d_time, b, hipMemcpyDeviceToHost, s->stream));
+    if (kin) HIPCHK(s, hipMemcpyAsync(kin, s->d_par1, b, hipMemcpyDefault, s->stream));
+    if (enst) HIPCHK(s, hipMemcpyAsync(enst, s->d_par2, b, hipMemcpyDefault, s->stream));
+    if (time) HIPCHK(s, hipMemcpyAsync(time, s->d_time, b, hipMemcpyDefault, s->stream));
     HIPCHK(s, hipStreamSynchronize(s->stream));
     return CNS_OK;
 }
@@ -1288,6 +1294,20 @@ extern "C" int cns_profile_read(cns_solver *s, double *total_ms, int *launches) 
     }
     *total_ms = tot;
     *launches = s->prof_n;
+    return CNS_OK;
+}
+
+extern "C" int cns_mem_info(int device, uint64_t *free_bytes, uint64_t *total_bytes) {
+    if (!free_bytes || !total_bytes) return fail(nullptr, CNS_ERR_ARG, "cns_mem_info: null argument");
+    int prev = 0;
+    if (hipGetDevice(&prev) != hipSuccess) return fail(nullptr, CNS_ERR_HIP, "cns_mem_info: no HIP device");
+    size_t f = 0, t = 0;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipMemGetInfo(&f, &t);
+    (void)hipSetDevice(prev);
+    if (e != hipSuccess) return fail(nullptr, CNS_ERR_HIP, std::string("cns_mem_info: ") + hipGetErrorString(e));
+    *free_bytes = f;
+    *total_bytes = t;
     return CNS_OK;
 }
 

@@ -40,6 +40,9 @@ struct Params {
     // (rho0, E0) with T0 = p0/(rho0 R), h0 = (E0 + p0)/rho0, p0 = (gam-1) E0;
     // all four are zero for FP64 storage
     double rho0, E0, T0, h0;
+    // FP32 storage, wall ghosts in offsets: wall temperature minus T0 at the
+    // bottom and top walls, and cp T0 - h0 (zero up to rounding)
+    double dtw_bot, dtw_top, ch0;
     double cF[4], cS[5], cVF[4], cVS[5];
     const double *xp, *vsx, *dxv, *spx, *spz;
     const double *ref;  // 5 sponge reference fields of mx*mz, contiguous
@@ -196,6 +199,14 @@ __device__ __forceinline__ void wall_ghost(const Params &P, R tw, R &r, R &u, R 
 __device__ __forceinline__ double recip(double x) { return 1.0 / x; }
 __device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
 
+// Viscosity law mu = T^n / Re for FP32 storage: 2^(n log2 T) with the
+// hardware base-2 logarithm and exponential (about 1 ulp each, so mu is
+// accurate to a few parts in 10^7, inside the 1e-5 budget of FP32 storage);
+// the FP64 path keeps pow.
+__device__ __forceinline__ float visc_law(float t, float n, float inv_re) {
+    return __builtin_amdgcn_exp2f(n * __builtin_amdgcn_logf(t)) * inv_re;
+}
+
 // Equation of state on offset storage (FP32): from rho' = rho - rho0 and
 // E' = rhoE - E0,
 //   p' = (gam-1)(E' - rho |u|^2/2),  T' = (p' - rho' R T0)/(rho R),
@@ -214,25 +225,30 @@ __device__ __forceinline__ void eos_off(const Params &P, R rp, R ru, R rv, R rw,
     const R pp = R(P.gam - 1.0) * (ep - r * k2);
     tp = (pp - rp * R(P.Rgas * P.T0)) * inv * R(1.0 / P.Rgas);
     hp = (ep + pp - rp * R(P.h0)) * inv;
-    mu = UNITV ? R(0) : R(pow(P.T0 + (double)tp, P.viscexp) * P.invRe);
+    mu = UNITV ? R(0) : visc_law(R(P.T0) + tp, R(P.viscexp), R(P.invRe));
 }
 
-// Wall ghost on offset storage, evaluated in FP64 on the full values.
+// Wall ghost on offset storage (FP32), from the mirror point's offsets:
+// with T_g = 2 Tw - T and p_g = p, rho_g = rho T / T_g, so
+//   T'_g   = 2 (Tw - T0) - T',
+//   rho'_g = (2 rho0 (T0 - Tw) + 2 rho0 T' + rho' T) / T_g,
+//   h'_g   = cp T'_g + |u|^2/2 + (cp T0 - h0),
+// every term of the size of the fluctuation, so FP32 keeps its relative
+// precision (dtw = Tw - T0 in FP64 on the host).
 template <typename R, bool UNITV>
-__device__ __forceinline__ void wall_ghost_off(const Params &P, double tw, R &rp, R &u, R &v, R &w, R &tp, R &hp,
+__device__ __forceinline__ void wall_ghost_off(const Params &P, R dtw, R &rp, R &u, R &v, R &w, R &tp, R &hp,
                                                R &mu) {
-    double r = P.rho0 + (double)rp, t = P.T0 + (double)tp;
-    const double p = r * P.Rgas * t;
-    const double ud = -(double)u, vd = -(double)v, wd = -(double)w;
-    t = 2.0 * tw - t;
-    r = p / (P.Rgas * t);
-    u = R(ud);
-    v = R(vd);
-    w = R(wd);
-    mu = UNITV ? R(0) : R(pow(t, P.viscexp) * P.invRe);
-    hp = R(t * P.cpR + 0.5 * (ud * ud + vd * vd + wd * wd) - P.h0);
-    rp = R(r - P.rho0);
-    tp = R(t - P.T0);
+    const R T0 = R(P.T0), r0 = R(P.rho0);
+    const R t = T0 + tp;
+    const R tpg = R(2) * dtw - tp;
+    const R tg = T0 + tpg;
+    rp = (R(2) * r0 * (tp - dtw) + rp * t) / tg;
+    u = -u;
+    v = -v;
+    w = -w;
+    tp = tpg;
+    hp = R(P.cpR) * tpg + R(0.5) * (u * u + v * v + w * w) + R(P.ch0);
+    mu = UNITV ? R(0) : visc_law(tg, R(P.viscexp), R(P.invRe));
 }
 
 // Quantity indices of the staged primitives (LDS plane and z register

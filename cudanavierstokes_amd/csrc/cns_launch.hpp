@@ -6,10 +6,11 @@
 
 namespace cns {
 
-// Tiles are TX x 16 columns.  The FP32 kernels with mu = T/Re run two waves
-// per SIMD, so a 512-lane workgroup (32 x 16 tile: 384 halo points for 512
-// interior ones instead of 256 for 256) fits; the others keep one wave per
-// SIMD, which allows 256 lanes (16 x 16).
+// Tiles are TX x 16 columns.  The FP32 kernels with mu = T/Re take 32 x 16
+// tiles in 512-lane workgroups (384 halo points for 512 interior ones
+// instead of 256 for 256); the other FP32 kernels 16 x 16 tiles in 256-lane
+// workgroups (32 x 16 measured slower for them); the FP64 role-split kernel
+// 16 x 16 tiles with two lanes per column (512 lanes).
 constexpr int TILE_Y = 16;
 template <typename R, bool UNITV>
 constexpr int tile_x() {

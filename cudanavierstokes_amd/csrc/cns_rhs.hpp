@@ -88,7 +88,7 @@ __device__ __forceinline__ void to_prim(const Params &P, const Raw<R> &x, int si
     R r = x.c[0], u, v, w, t, h, mu;
     if constexpr (Off<R>::on) {
         eos_off<R, UNITV>(P, r, x.c[1], x.c[2], x.c[3], x.c[4], u, v, w, t, h, mu);
-        if (WALLX && side != 0) wall_ghost_off<R, UNITV>(P, side < 0 ? P.twall_bot : P.twall_top, r, u, v, w, t, h, mu);
+        if (WALLX && side != 0) wall_ghost_off<R, UNITV>(P, R(side < 0 ? P.dtw_bot : P.dtw_top), r, u, v, w, t, h, mu);
     } else {
         eos<R, UNITV>(P, r, x.c[1], x.c[2], x.c[3], x.c[4], u, v, w, t, h, mu);
         if (WALLX && side != 0)
@@ -1002,19 +1002,18 @@ __device__ __forceinline__ void rhs_tile_split(const Params &P, const StagePtrs<
 }
 
 // Kernel form per storage type: FP64 runs the role-split tile (two waves
-// per SIMD, each role within 256 registers); FP32 runs rhs_tile.
+// per SIMD, each role within 256 registers); FP32 runs rhs_tile, which fits
+// 256 registers on its own (the role split measured slower there).
 template <typename R, bool UNITV>
 constexpr bool rhs_split() {
     return sizeof(R) == 8;
 }
-// Waves per SIMD the RHS kernel is compiled for: the role-split kernel and
-// FP32 storage with mu = T/Re fit two (at most 256 VGPRs; the latter with a
-// few scratch spills, which cost less than the latency the second wave
-// hides); FP32 with a general viscosity law needs more registers than two
-// waves can have and keeps one.
+// Waves per SIMD the RHS kernel is compiled for: two in every form (at most
+// 256 VGPRs; FP32 with mu = T/Re with a few scratch spills, which cost less
+// than the latency the second wave hides).
 template <typename R, bool UNITV>
 constexpr int rhs_waves() {
-    return (rhs_split<R, UNITV>() || (sizeof(R) == 4 && UNITV)) ? 2 : 1;
+    return 2;
 }
 // Lanes of one RHS workgroup for a TX x TY tile.
 template <typename R, bool UNITV, int TX, int TY>

@@ -113,7 +113,8 @@ int cns_get_state(cns_solver *s, double *r, double *u, double *v, double *w, dou
 int cns_rhs(cns_solver *s, double *rr, double *ru, double *rv, double *rw, double *re);
 
 /* replaces runSimulation[LowStorage] (src/cuda_main.cu:44-186): n_steps
- * steps.  Optional host outputs of length n_steps: kin/enst receive the
+ * steps.  Optional outputs of length n_steps, host or device arrays (the
+ * reference passes device arrays, src/main.h:35): kin/enst receive the
  * reference's par1/par2 bulk statistics at steps with istep % check_bulk == 0
  * (NaN elsewhere; par2 is NaN without forcing), time the simulation time
  * after each step. */
@@ -217,6 +218,10 @@ int cns_tune(cns_solver *s, int *pencil);
 
 /* bytes of device memory held by the solver */
 int cns_device_bytes(const cns_solver *s, uint64_t *bytes);
+
+/* free and total device memory of a device (the query of checkGpuMem,
+ * src/cuda_utils.cu:780-813) */
+int cns_mem_info(int device, uint64_t *free_bytes, uint64_t *total_bytes);
 
 /* ---- Batched advance of many solvers (ensembles, the serving loop) ----
  * No counterpart in the reference, which runs one flow per process.  A batch

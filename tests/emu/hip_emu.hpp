@@ -70,6 +70,9 @@ inline void __builtin_amdgcn_raw_buffer_store_b32(unsigned v, __amdgpu_buffer_rs
 
 // hardware reciprocal: IEEE division on the CPU
 inline float __builtin_amdgcn_rcpf(float x) { return 1.0f / x; }
+// hardware base-2 logarithm and exponential: the C library's on the CPU
+inline float __builtin_amdgcn_logf(float x) { return std::log2(x); }
+inline float __builtin_amdgcn_exp2f(float x) { return std::exp2(x); }
 
 // scheduling hints and wave-uniform reads have no effect on the CPU
 #define __builtin_amdgcn_sched_barrier(m) ((void)(m))
@@ -106,12 +109,21 @@ typedef int hipError_t;
 typedef void *hipStream_t;
 typedef void *hipEvent_t;
 enum { hipSuccess = 0, hipErrorInvalidValue = 1, hipErrorOutOfMemory = 2 };
-enum hipMemcpyKind { hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice };
+enum hipMemcpyKind { hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice, hipMemcpyDefault };
 #define hipStreamNonBlocking 1
 
 inline const char *hipGetErrorString(hipError_t e) { return e ? "emulated error" : "no error"; }
 inline hipError_t hipGetLastError() { return hipSuccess; }
 inline hipError_t hipSetDevice(int) { return hipSuccess; }
+inline hipError_t hipGetDevice(int *d) {
+    *d = 0;
+    return hipSuccess;
+}
+// a nominal 288 GB device with nothing allocated
+inline hipError_t hipMemGetInfo(size_t *free_b, size_t *total_b) {
+    *free_b = *total_b = (size_t)288 << 30;
+    return hipSuccess;
+}
 inline hipError_t hipGetDeviceCount(int *n) {
     *n = 1;
     return hipSuccess;

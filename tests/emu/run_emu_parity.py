@@ -27,23 +27,35 @@ CASES = [
                              nonuniform_x=1, forcing=1, Lx=2.0, Ma=1.5, Re=3000.0, viscexp=0.7, check_cfl=1), 1),
 ]
 
-worst = 0.0
-for what, kind, kw, ns in CASES:
-    cfg = cns.Config(**kw)
-    orc = oracle_from(cfg)
-    f = initial_fields(kind, orc)
-    orc.set_state(*f)
-    sol = cns.Solver(cfg)
-    sol.set_state(*f)
-    if what == "rhs":
-        e = cases.rel_errors(sol.rhs(), orc.rhs())
-    else:
-        orc.step(ns)
-        sol.step(ns)
-        e = state_errors(sol.get_state(), orc.get_state())
-    sol.close()
-    m = max(v[0] for v in e.values())
-    worst = max(worst, m)
-    print(what, kind, kw.get("scheme", "-"), f"{m:.2e}", flush=True)
-print("WORST", worst)
-sys.exit(0 if worst <= 1e-12 else 1)
+# FP32 storage against the FP64 oracle (relative L2 <= 1e-5): offset EOS,
+# FP32 viscosity law and wall ghosts, compensated RK sums
+CASES32 = [
+    ("step", "tgv", dict(mx=12, my=12, mz=8, stencil_size=2, stencil_visc=2, scheme=2), 2),
+    ("step", "channel", dict(mx=14, my=8, mz=8, stencil_size=3, stencil_visc=2, periodic_x=0, nonuniform_x=1,
+                             forcing=1, Lx=2.0, Ma=1.5, Re=3000.0, viscexp=0.7, scheme=0, check_cfl=1), 2),
+    ("step", "channel", dict(mx=10, my=8, mz=8, stencil_size=2, stencil_visc=2, periodic_x=0, nonuniform_x=0,
+                             Lx=2.0, Ma=0.8, Re=500.0, viscexp=0.7, scheme=1, twall_bot=1.1, twall_top=0.9), 2),
+]
+
+bad = []
+for precision, cases_, tol in ((0, CASES, 1e-12), (1, CASES32, 1e-5)):
+    for what, kind, kw, ns in cases_:
+        cfg = cns.Config(**kw)
+        orc = oracle_from(cfg)
+        f = initial_fields(kind, orc)
+        orc.set_state(*f)
+        sol = cns.Solver(cfg.replace(precision=precision))
+        sol.set_state(*f)
+        if what == "rhs":
+            e = cases.rel_errors(sol.rhs(), orc.rhs())
+        else:
+            orc.step(ns)
+            sol.step(ns)
+            e = state_errors(sol.get_state(), orc.get_state())
+        sol.close()
+        m = max(v[0] for v in e.values())
+        print(what, kind, kw.get("scheme", "-"), "fp32" if precision else "fp64", f"{m:.2e}", flush=True)
+        if not m <= tol:
+            bad.append((what, kind, precision, m))
+print("FAILED" if bad else "OK", bad)
+sys.exit(1 if bad else 0)
