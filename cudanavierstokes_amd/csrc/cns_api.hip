@@ -1242,6 +1242,90 @@ extern "C" int cns_time_steps(cns_solver *s, int nsteps, double *ms) {
     return CNS_OK;
 }
 
+// Natural cubic spline through (xk, yk) (zero second derivative at both
+// ends), the interpolation of calculateSponge (src/sponge.cu:258-313): the
+// knots' second derivatives from the tridiagonal system, then the cubic of
+// the interval holding each abscissa (the end intervals' cubics continue
+// outside the knots).
+namespace {
+struct NaturalSpline {
+    const double *xk, *yk;
+    int n;
+    std::vector<double> m;
+    NaturalSpline(const double *x, const double *y, int count) : xk(x), yk(y), n(count), m(count, 0.0) {
+        if (n < 3) return;
+        const int k = n - 2;
+        std::vector<double> cp(k), dp(k);
+        for (int i = 0; i < k; i++) {
+            const double h0 = xk[i + 1] - xk[i], h1 = xk[i + 2] - xk[i + 1];
+            const double b = 2.0 * (h0 + h1);
+            const double d = 6.0 * ((yk[i + 2] - yk[i + 1]) / h1 - (yk[i + 1] - yk[i]) / h0);
+            const double den = i ? b - h0 * cp[i - 1] : b;
+            cp[i] = i < k - 1 ? h1 / den : 0.0;
+            dp[i] = i ? (d - h0 * dp[i - 1]) / den : d / den;
+        }
+        m[k] = dp[k - 1];
+        for (int i = k - 2; i >= 0; i--) m[i + 1] = dp[i] - cp[i] * m[i + 2];
+    }
+    double operator()(double x) const {
+        int lo = 0, hi = n - 1;  // largest knot <= x, clamped to the first and last interval
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) / 2;
+            if (xk[mid] <= x)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        const double h = xk[lo + 1] - xk[lo];
+        const double a = (xk[lo + 1] - x) / h, b = (x - xk[lo]) / h;
+        return a * yk[lo] + b * yk[lo + 1] + ((a * a * a - a) * m[lo] + (b * b * b - b) * m[lo + 1]) * h * h / 6.0;
+    }
+};
+}  // namespace
+
+extern "C" int cns_sponge_from_profile(int n, const double *yp, const double *rp, const double *up, const double *wp,
+                                       int mx, const double *x, int mz, const double *z, double len_x, double len_z,
+                                       double reynolds, double gamma, double mach, const double *zones,
+                                       double *sigma_x, double *sigma_z, double *rref, double *ruref, double *rvref,
+                                       double *rwref, double *reref) {
+    if (n < 2 || mx < 1 || mz < 1 || !yp || !rp || !up || !wp || !x || !z || !sigma_x || !sigma_z || !rref ||
+        !ruref || !rvref || !rwref || !reref)
+        return fail(nullptr, CNS_ERR_ARG, "cns_sponge_from_profile: bad argument");
+    for (int i = 1; i < n; i++)
+        if (!(yp[i] > yp[i - 1])) return fail(nullptr, CNS_ERR_ARG, "cns_sponge_from_profile: profile abscissae must increase");
+    // src/sponge.h:4-17: (strength, length, exponent) of the top, inlet and outlet zones
+    static const double ref_zones[9] = {1.0, 1.0, 2.0, 0.5, 20.0, 2.0, 0.5, 20.0, 2.0};
+    const double *zn = zones ? zones : ref_zones;
+    // src/sponge.cu:112-129; the outlet zone wins where inlet and outlet overlap
+    for (int i = 0; i < mx; i++) {
+        sigma_x[i] = 0.0;
+        if (zn[1] > 0.0 && x[i] >= len_x - zn[1]) sigma_x[i] = zn[0] * std::pow((x[i] - (len_x - zn[1])) / zn[1], zn[2]);
+    }
+    for (int k = 0; k < mz; k++) {
+        sigma_z[k] = 0.0;
+        if (zn[4] > 0.0 && z[k] <= zn[4]) sigma_z[k] = zn[3] * std::pow((zn[4] - z[k]) / zn[4], zn[5]);
+        if (zn[7] > 0.0 && z[k] >= len_z - zn[7]) sigma_z[k] = zn[6] * std::pow((z[k] - (len_z - zn[7])) / zn[7], zn[8]);
+    }
+    // src/sponge.cu:158-168: profile read at x / sqrt(1 + z/Re), wall-normal
+    // velocity over (scale Re), pressure Rgas; conserved variables
+    const NaturalSpline sr(yp, rp, n), su(yp, up, n), sw(yp, wp, n);
+    const double rgas = 1.0 / (gamma * mach * mach);
+    for (int k = 0; k < mz; k++) {
+        const double scale = std::sqrt(1.0 + z[k] / reynolds);
+        for (int i = 0; i < mx; i++) {
+            const size_t g = i + (size_t)k * mx;
+            const double xs = x[i] / scale;
+            const double r = sr(xs), u = su(xs) / (scale * reynolds), w = sw(xs);
+            rref[g] = r;
+            ruref[g] = r * u;
+            rvref[g] = 0.0;
+            rwref[g] = r * w;
+            reref[g] = rgas / (gamma - 1.0) + r * 0.5 * (u * u + w * w);
+        }
+    }
+    return CNS_OK;
+}
+
 extern "C" int cns_set_sponge(cns_solver *s, const double *sigma_x, const double *sigma_z, const double *rref,
                               const double *ruref, const double *rvref, const double *rwref, const double *reref) {
     if (!s || !sigma_x || !sigma_z || !rref || !ruref || !rvref || !rwref || !reref)

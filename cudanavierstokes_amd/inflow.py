@@ -166,6 +166,16 @@ def reference_fields(x, z, prof, Re, gam, Ma):
     return [r, r * u, np.zeros_like(r), r * w, e]
 
 
+def write_profile_files(directory, prof):
+    """The profile as the reference's blasius1D/{x,r,u,w,e}Prof.bin (raw
+    float64; x holds y / delta_99), the input calculateSponge reads
+    (src/sponge.cu:131-156; written by python-utils/selfSimilarSol.py)."""
+    import os
+    os.makedirs(directory, exist_ok=True)
+    for name, key in (("x", "y"), ("r", "r"), ("u", "u"), ("w", "w"), ("e", "e")):
+        np.ascontiguousarray(prof[key], dtype=np.float64).tofile(os.path.join(directory, f"{name}Prof.bin"))
+
+
 def calculate_sponge(solver, prof=None, set_state=False, **zones):
     """calculateSponge for a solver: strengths from the zone constants
     (keywords top, inlet, outlet override src/sponge.h), reference state from
@@ -186,4 +196,4 @@ def calculate_sponge(solver, prof=None, set_state=False, **zones):
 
 
 __all__ = ["SPONGE_TOP", "SPONGE_INLET", "SPONGE_OUTLET", "sponge_strengths", "similarity_profile", "natural_spline",
-           "reference_fields", "calculate_sponge"]
+           "reference_fields", "write_profile_files", "calculate_sponge"]

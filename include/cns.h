@@ -120,6 +120,21 @@ int cns_rhs(cns_solver *s, double *rr, double *ru, double *rv, double *rw, doubl
  * after each step. */
 int cns_step(cns_solver *s, int n_steps, double *kin, double *enst, double *time);
 
+/* sponge strengths and reference state of calculateSponge
+ * (src/sponge.cu:83-168) from a boundary-layer profile on y / delta
+ * (n points, increasing: the layout of blasius1D/{x,r,u,w}Prof.bin: density,
+ * wall-normal and streamwise velocity), interpolated by a natural cubic
+ * spline at x / sqrt(1 + z/Re) on the grid x_coord[n_x], z_coord[n_z].  zones: 9 values
+ * (strength, length, exponent) of the top, inlet and outlet zones, or NULL
+ * for src/sponge.h:4-17.  Outputs sigma_x[n_x], sigma_z[n_z] and the
+ * reference conserved fields [n_z][n_x] for cns_set_sponge.  Host
+ * computation only. */
+int cns_sponge_from_profile(int n, const double *y, const double *rho, const double *u_wallnormal,
+                            const double *w_streamwise, int n_x, const double *x_coord, int n_z,
+                            const double *z_coord, double len_x, double len_z, double reynolds, double gamma,
+                            double mach, const double *zones, double *sigma_x, double *sigma_z, double *rref,
+                            double *ruref, double *rvref, double *rwref, double *reref);
+
 /* sponge of addSponge (src/sponge.cu:31-41): strengths sigma_x[mx],
  * sigma_z[mz] and reference conserved fields [mz][mx] */
 int cns_set_sponge(cns_solver *s, const double *sigma_x, const double *sigma_z, const double *rref,

@@ -71,6 +71,56 @@ def test_compat_shim_compiles_after_reference_macros(tmp_path):
     assert r.returncode == 0, r.stderr
 
 
+BL_CASE = CASE.replace("#define boundaryLayer (false)", "#define boundaryLayer (true)").replace(
+    "#define forcing (true)", "#define forcing (false)").replace(
+    "double r[mx*my*mz]", "double x[mx], z[mz_tot];\ndouble r[mx*my*mz]").replace(
+    "#include \"cns_compat.hpp\"",
+    "void writeField(int, Communicator) {}\nvoid calcAvgChan(Communicator) {}\n#include \"cns_compat.hpp\"").replace(
+    "    setDevice(0);", "    calculateSponge(rk);\n    setDevice(0);").replace(
+    "    runSimulation(kin, enst, time, rk);", "    checkGpuMem(rk);\n    solverWrapper(rk);\n    runSimulationLowStorage(kin, enst, time, rk);")
+
+
+def test_compat_shim_compiles_for_a_boundary_layer_main(tmp_path):
+    """calculateSponge, checkGpuMem, solverWrapper and runSimulationLowStorage
+    instantiate in a boundary-layer case with the host routines a main.cpp
+    program provides."""
+    src = tmp_path / "main_bl.cpp"
+    src.write_text(BL_CASE)
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), str(src)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def test_sponge_from_profile_matches_inflow():
+    """cns_sponge_from_profile (C++, the routine behind calculateSponge) gives
+    the sponge strengths and reference state of cudanavierstokes_amd.inflow
+    bitwise (host computation: runs without a GPU)."""
+    import numpy as np
+
+    from cudanavierstokes_amd import _lib, inflow
+    lib = _lib.load()
+    prof = inflow.similarity_profile(gam=1.4, Ma=0.3, Pr=0.72, n=200, n_out=300)
+    mx, mz = 40, 96
+    x = np.sort(np.random.default_rng(1).uniform(0.0, 8.0, mx))
+    z = np.linspace(0.5, 300.0, mz)
+    Lx, Lz, Re = 8.0, 300.0, 500.0
+    sx, sz = np.zeros(mx), np.zeros(mz)
+    refs = [np.zeros(mx * mz) for _ in range(5)]
+    rc = lib.cns_sponge_from_profile(len(prof["y"]), prof["y"], prof["r"], prof["u"], prof["w"], mx, x, mz, z, Lx,
+                                     Lz, Re, 1.4, 0.3, None, sx, sz, *refs)
+    assert rc == 0
+    sx2, sz2 = inflow.sponge_strengths(x, z, Lx, Lz)
+    assert np.array_equal(sx, sx2) and np.array_equal(sz, sz2)
+    assert np.count_nonzero(sx) > 0 and np.count_nonzero(sz) > 0
+    for a, b in zip(refs, inflow.reference_fields(x, z, prof, Re, 1.4, 0.3)):
+        assert np.array_equal(a.reshape(mz, mx), b)
+    # a profile whose abscissae do not increase is an error, not a crash
+    bad = prof["y"].copy()
+    bad[5] = bad[4]
+    assert lib.cns_sponge_from_profile(len(bad), bad, prof["r"], prof["u"], prof["w"], mx, x, mz, z, Lx, Lz, Re,
+                                       1.4, 0.3, None, sx, sz, *refs) != 0
+
+
 def test_c_header_compiles_as_c(tmp_path):
     src = tmp_path / "t.c"
     src.write_text('#include "cns.h"\nint main(void){cns_config c; return cns_config_init(&c);}\n')

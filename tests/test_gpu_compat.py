@@ -1,8 +1,10 @@
-"""The reference's call sequence from C++: tests/compat/main_like.cpp, a
-main.cpp-shaped driver with src/globals.h-style case macros and host arrays,
-drives libcns.so through include/cns_compat.hpp (setDevice,
-setGPUParameters, initSolver, copyField, runSimulation, clearSolver).  Its
-result is bitwise the result of the same run through the Python binding."""
+"""The reference's call sequence from C++ over include/cns_compat.hpp and
+libcns.so, bitwise against the same run through the Python binding:
+  - tests/compat/main_like.cpp: setDevice, setGPUParameters, initSolver,
+    copyField, runSimulation, clearSolver with host arrays;
+  - tests/compat/main_ref.cpp: src/main.cpp:22-79 call for call (MPI stubbed
+    for one rank), solverWrapper's nfiles loop with its field files and
+    solution.txt, checkGpuMem."""
 import json
 import os
 import subprocess
@@ -44,3 +46,37 @@ def test_main_like_driver_through_the_shim(tmp_path):
     for a, b in zip(got, final):
         assert np.array_equal(a, b)
     assert info["time"] == time[-1] and info["kin0"] == kin[0]
+
+
+REF = os.path.join(ROOT, "tests", "compat", "main_ref")
+
+
+def test_reference_main_through_the_shim(tmp_path):
+    if not os.path.exists(REF):
+        pytest.fail(f"{REF} is not built (run __graft_entry__.build())")
+    flush_print("[case] src/main.cpp-shaped driver: 32^3 order 4 SMR RK3, 2 files x 10 steps")
+    (tmp_path / "fields").mkdir()
+    r = subprocess.run([REF], capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert "Memory used:" in r.stdout and "Simulation is finished!" in r.stdout
+    n = 32 ** 3
+
+    def fields(step):
+        return [np.fromfile(tmp_path / "fields" / f"{q}.{step:07d}.bin", dtype=np.float64).reshape(32, 32, 32)
+                for q in "ruvwe"]
+
+    init = fields(0)
+    cfg = cns.taylor_green(n=32, order=4, scheme="rk3", cfl=0.5, check_cfl=10, check_bulk=10, Re=1600.0,
+                           Pr=0.71, Ma=0.1, viscexp=1.0)
+    with cns.Solver(cfg) as s:
+        s.set_state(*init)
+        for f in (1, 2):
+            kin, enst, time = s.step(10)
+            got = s.get_state()
+            for a, b in zip(got, fields(f)):
+                assert a.size == n and np.array_equal(a, b), f
+    # solution.txt: one line per file (CFL step 0 of each file's 10 steps)
+    lines = (tmp_path / "solution.txt").read_text().split()
+    assert len(lines) == 2 * 5
+    assert float(lines[5 + 1]) == pytest.approx(time[0], rel=1e-6)
+    assert (tmp_path / "prof.txt").exists()
