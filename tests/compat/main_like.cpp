@@ -26,6 +26,8 @@
 #define my my_tot
 #define mz mz_tot
 #define nsteps 20
+#define nfiles 1
+#define restartFile -1
 #define CFL 0.5
 #define lowStorage (true)
 #define boundaryLayer (false)

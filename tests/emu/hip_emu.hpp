@@ -24,6 +24,9 @@
 #define __launch_bounds__(...)
 #define __shared__ static
 #define __restrict__
+#define __align__(n) alignas(n)
+// a device symbol is an ordinary host object on the CPU
+#define HIP_SYMBOL(sym) (&(sym))
 
 struct dim3 {
     unsigned x, y, z;
@@ -115,6 +118,10 @@ enum hipMemcpyKind { hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDevi
 inline const char *hipGetErrorString(hipError_t e) { return e ? "emulated error" : "no error"; }
 inline hipError_t hipGetLastError() { return hipSuccess; }
 inline hipError_t hipSetDevice(int) { return hipSuccess; }
+inline hipError_t hipGetSymbolAddress(void **p, const void *sym) {
+    *p = const_cast<void *>(sym);
+    return hipSuccess;
+}
 inline hipError_t hipGetDevice(int *d) {
     *d = 0;
     return hipSuccess;
