@@ -48,7 +48,7 @@ C_NAMES = {"mx": "nx", "my": "ny", "mz": "nz", "forcing": "forcing_z", "Lx": "le
 
 EXPORTS = [
     "cns_config_init", "cns_create", "cns_destroy", "cns_last_error", "cns_get_grid",
-    "cns_fill_taylor_green", "cns_fill_channel", "cns_init_taylor_green", "cns_init_channel", "cns_set_state", "cns_get_state", "cns_rhs",
+    "cns_fill_taylor_green", "cns_fill_channel", "cns_init_taylor_green", "cns_init_channel", "cns_set_state", "cns_get_state", "cns_rhs", "cns_rhs_noalloc",
     "cns_step", "cns_set_sponge", "cns_get_scalars", "cns_sync", "cns_profile_enable",
     "cns_profile_read", "cns_time_steps", "cns_device_bytes", "cns_mem_info", "cns_sponge_from_profile",
     "cns_write_fields", "cns_read_fields", "cns_write_grid", "cns_get_control", "cns_set_control",
@@ -89,6 +89,7 @@ def load(path=None):
     L.cns_set_state.argtypes = [vp, dp, dp, dp, dp, dp]
     L.cns_get_state.argtypes = [vp, dp, dp, dp, dp, dp]
     L.cns_rhs.argtypes = [vp, dp, dp, dp, dp, dp]
+    L.cns_rhs_noalloc.argtypes = [vp, vp, vp]
     L.cns_step.argtypes = [vp, C.c_int, vp, vp, vp]
     L.cns_set_sponge.argtypes = [vp, dp, dp, dp, dp, dp, dp, dp]
     L.cns_get_scalars.argtypes = [vp, vp, vp, vp]

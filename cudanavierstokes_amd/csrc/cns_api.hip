@@ -24,6 +24,7 @@
 #include <cstring>
 #include <string>
 #include <thread>
+#include <mutex>
 #include <vector>
 
 #include "../../include/cns.h"
@@ -862,13 +863,15 @@ extern "C" int cns_get_state(cns_solver *s, double *r, double *u, double *v, dou
     return CNS_OK;
 }
 
-// dilatation + fused RHS/update of one stage
+// dilatation + fused RHS/update of one stage on the stage input qin, with
+// the dilatation field in dil; with timing, the RHS launch is bracketed by
+// events when profiling is on
 template <typename R>
-static int run_stage(cns_solver *s, int in_set, Params P, StagePtrs<R> A) {
+static int run_stage_on(cns_solver *s, const R *qin, R *dil, Params P, StagePtrs<R> A, bool timing) {
     const bool wallx = !s->cfg.periodic_x, nonuni = s->cfg.nonuniform_x != 0;
-    A.qin = as<R>(s->sets[in_set]);
+    A.qin = qin;
     A.fs = s->n;
-    A.dil = as<R>(s->d_dil);
+    A.dil = dil;
     fill_unused(A);
     // the dilatation kernel needs few registers, so it always runs 512-lane
     // workgroups on 32 x 16 tiles (fewer halo points per point), with its own
@@ -889,13 +892,14 @@ static int run_stage(cns_solver *s, int in_set, Params P, StagePtrs<R> A) {
         Pd = P;
         dil_blocks = s->nblocks;
     }
-    HIPCHK(s, launch_dil<R>(s->SV, wallx, nonuni, dil_blocks, s->stream, Pd, A, as<R>(s->d_dil)));
+    HIPCHK(s, launch_dil<R>(s->SV, wallx, nonuni, dil_blocks, s->stream, Pd, A, dil));
     LAUNCHED(s, "k_dil");
     if (P.want_visc_max) {
         hipLaunchKernelGGL(k_reset_visc, dim3(1), dim3(1), 0, s->stream, s->d_sc);
         LAUNCHED(s, "k_reset_visc");
     }
-    if (s->prof) {
+    const bool prof = s->prof && timing;
+    if (prof) {
         const size_t need = 2 * (size_t)(s->prof_n + 1);
         while (s->ev.size() < need) {
             hipEvent_t e;
@@ -906,11 +910,16 @@ static int run_stage(cns_solver *s, int in_set, Params P, StagePtrs<R> A) {
     }
     HIPCHK(s, launch_rhs<R>(s->S, s->SV, wallx, nonuni, s->nblocks, s->stream, P, A));
     LAUNCHED(s, "k_rhs");
-    if (s->prof) {
+    if (prof) {
         HIPCHK(s, hipEventRecord(s->ev[2 * s->prof_n + 1], s->stream));
         s->prof_n++;
     }
     return CNS_OK;
+}
+
+template <typename R>
+static int run_stage(cns_solver *s, int in_set, Params P, StagePtrs<R> A) {
+    return run_stage_on<R>(s, as<R>(s->sets[in_set]), as<R>(s->d_dil), P, A, true);
 }
 
 template <typename R>
@@ -939,6 +948,65 @@ static int rhs_t(cns_solver *s, double *const dst[5]) {
             for (long g = 0; g < n; g++) dst[q][g] = (double)h[q * n + g];
     }
     return CNS_OK;
+}
+
+// Scratch of cns_rhs_noalloc: the dilatation field of one evaluation, in a
+// file-scope device array (16 MiB on every GPU the library runs on, held
+// until the process ends), so that the entry point allocates nothing.  Host
+// code reaches it only through the per-device address hipGetSymbolAddress
+// returns (the array's name on the host is a stand-in, not a device
+// address).
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)16 << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[NOALLOC_MAX_DEVICES];
+}  // namespace
+
+// Device address of the scratch array on device dev (current), looked up once
+// per device.
+static int scratch_address(cns_solver *s, int dev, void **out) {
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(s, CNS_ERR_UNSUPPORTED, "cns_rhs_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(s, CNS_ERR_HIP, std::string("cns_rhs_noalloc: hipGetSymbolAddress(scratch): ") +
+                                            hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return CNS_OK;
+}
+
+template <typename R>
+static int rhs_noalloc_t(cns_solver *s, const void *q, void *rhs, void *dil) {
+    Params P = s->base;
+    P.write_q = 0;
+    P.write_r = 1;
+    P.want_visc_max = 0;
+    StagePtrs<R> A;
+    std::memset(&A, 0, sizeof(A));
+    A.rout = static_cast<R *>(rhs);
+    return run_stage_on<R>(s, static_cast<const R *>(q), static_cast<R *>(dil), P, A, false);
+}
+
+extern "C" int cns_rhs_noalloc(cns_solver *s, const void *q, void *rhs) {
+    if (!s || !q || !rhs) return fail(s, CNS_ERR_ARG, "cns_rhs_noalloc: null argument");
+    if (s->in_batch) return fail(s, CNS_ERR_STATE, "cns_rhs_noalloc: the solver is advanced by a batch");
+    const size_t need = (size_t)s->n * (s->fp32 ? sizeof(float) : sizeof(double));
+    if (need > NOALLOC_SCRATCH_BYTES)
+        return fail(s, CNS_ERR_ARG, "cns_rhs_noalloc: " + std::to_string(s->n) + " points need " + std::to_string(need) +
+                                        " bytes of scratch, above its maximum of " +
+                                        std::to_string(NOALLOC_SCRATCH_BYTES) + " bytes");
+    HIPCHK(s, hipSetDevice(s->dev));
+    void *dil = nullptr;
+    int rc = scratch_address(s, s->dev, &dil);
+    if (rc) return rc;
+    return s->fp32 ? rhs_noalloc_t<float>(s, q, rhs, dil) : rhs_noalloc_t<double>(s, q, rhs, dil);
 }
 
 extern "C" int cns_rhs(cns_solver *s, double *rr, double *ru, double *rv, double *rw, double *re) {

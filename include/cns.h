@@ -112,6 +112,19 @@ int cns_get_state(cns_solver *s, double *r, double *u, double *v, double *w, dou
  * src/cuda_main.cu:15-42), five host arrays of the conserved-variable RHS */
 int cns_rhs(cns_solver *s, double *rr, double *ru, double *rv, double *rw, double *re);
 
+/* No counterpart in the reference: cns_rhs for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid allocation).  One right-hand side of the state q into rhs, both
+ * device arrays the caller owns in the solver's device layout (five fields
+ * of nx*ny*nz values: rho, rho u, rho v, rho w, rho E; FP64, or for FP32
+ * storage the FP32 offsets from the solver's reference state), queued on the
+ * solver's stream (cns_sync waits).  Allocates nothing: the dilatation field
+ * goes to a file-scope device scratch array of 16 MiB per GPU (held until
+ * the process ends), so grids above 2^21 FP64 or 2^22 FP32 points are
+ * CNS_ERR_ARG.  Every call on one GPU shares that array: calls that overlap
+ * on one GPU (other streams, other threads) race. */
+int cns_rhs_noalloc(cns_solver *s, const void *q, void *rhs);
+
 /* replaces runSimulation[LowStorage] (src/cuda_main.cu:44-186): n_steps
  * steps.  Optional outputs of length n_steps, host or device arrays (the
  * reference passes device arrays, src/main.h:35): kin/enst receive the
