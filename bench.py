@@ -37,21 +37,14 @@ def parse_args():
     ap.add_argument("--other-steps", type=int, default=20, help="steps per other reference configuration (0: skip)")
     ap.add_argument("--err-steps", type=int, default=20, help="steps of the FP32-vs-FP64 error comparison")
     ap.add_argument("--ensemble-steps", type=int, default=100, help="steps of the 64-instance ensemble (0: skip)")
-    ap.add_argument("--serve-requests", type=int, default=100,
+    ap.add_argument("--serve-requests", type=int, default=1000,
                     help="requests of the seeded serving stream to run (0: skip; the full stream has 1000)")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0: host cores, at most 16)")
+    ap.add_argument("--cpu-c2-single", action="store_true",
+                    help="also time one 256^3 oracle step on a single thread (about half a minute)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (reference point for the "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counters) and exit")
     return ap.parse_args()
-
-
-def host_cores():
-    try:
-        n = len(os.sched_getaffinity(0))
-    except Exception:
-        n = os.cpu_count() or 1
-    return max(1, min(16, n))
 
 
 def read_traffic():
@@ -68,22 +61,98 @@ def read_traffic():
     return None
 
 
-def cpu_baseline(n, order, threads):
-    """The FP64 oracle (a CPU restatement of the reference algorithm), one
-    RK4 step of the same 256^3 8th-order Taylor-Green case."""
+def cpu_topology():
+    """CPU model, the CPUs this process may run on, and the physical cores
+    among them (lscpu's fields, read from /proc and /sys)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        cpus = sorted(os.sched_getaffinity(0))
+    except Exception:
+        cpus = list(range(os.cpu_count() or 1))
+    cores = set()
+    for c in cpus:
+        try:
+            base = f"/sys/devices/system/cpu/cpu{c}/topology/"
+            with open(base + "physical_package_id") as f:
+                pkg = f.read().strip()
+            with open(base + "core_id") as f:
+                core = f.read().strip()
+            cores.add((pkg, core))
+        except OSError:
+            cores.add(("?", c))
+    return {"model": model, "cpus_allowed": len(cpus), "physical_cores": len(cores)}
+
+
+def cpu_baseline(full_c2_single=False):
+    """The FP64 oracle (a CPU restatement of the reference algorithm; the
+    reference has no CPU build) on the host's cores, as planned for the
+    baseline: one warm-up step, then the median of the repetitions of the
+    step loop alone (steady clock), on 1 thread and on all physical cores
+    (OMP_PROC_BIND=close, OMP_PLACES=cores):
+      C1  64^3 order 4 low-storage RK3, the full 100 steps (5 repetitions on
+          all cores; one repetition of 10 steps on 1 thread)
+      C2  256^3 order 8 classic RK4, per-step rate (3 single steps on all
+          cores; on 1 thread only with full_c2_single, about half a minute)
+      C3  160x128x192 order 6 channel, per-step rate (3 single steps on all
+          cores, 1 on 1 thread)
+    value: C2 on all cores (the headline workload), Gpt-RKsteps/s."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     from oracle import oracle as O
-    from tests.cases import tgv_fields
-    O.set_threads(threads)
-    orc = O.Oracle(mx=n, my=n, mz=n, s=order // 2, sv=order // 2, scheme="rk4", cfl=0.5, check_cfl=10,
-                   check_bulk=10, Re=1600.0, Pr=0.71, Ma=0.1)
-    orc.set_state(*tgv_fields(orc.grid(), orc.cfg))
-    t0 = time.perf_counter()
-    orc.step(1)
-    dt = time.perf_counter() - t0
-    orc.close()
-    return {"value": n ** 3 / dt / 1e9, "unit": "Gpt-RKsteps/s", "cores": threads, "kind": "port",
-            "sample": f"1 RK4 step of the {n}^3 order-{order} Taylor-Green case on {threads} host threads "
-                      f"({dt:.2f} s)"}
+    from tests.cases import channel_fields, tgv_fields
+    topo = cpu_topology()
+    allc = max(1, topo["physical_cores"])
+
+    def make(name):
+        if name == "C1":
+            o = O.Oracle(mx=64, my=64, mz=64, s=2, sv=2, scheme="smr_rk3", cfl=0.5, check_cfl=10, check_bulk=10,
+                         Re=1600.0, Pr=0.71, Ma=0.1)
+            o.set_state(*tgv_fields(o.grid(), o.cfg))
+        elif name == "C2":
+            o = O.Oracle(mx=256, my=256, mz=256, s=4, sv=4, scheme="rk4", cfl=0.5, check_cfl=10, check_bulk=10,
+                         Re=1600.0, Pr=0.71, Ma=0.1)
+            o.set_state(*tgv_fields(o.grid(), o.cfg))
+        else:
+            o = O.Oracle(mx=160, my=128, mz=192, s=3, sv=3, scheme="smr_rk3", periodic_x=0, nonuniform_x=1,
+                         forcing=1, Lx=2.0, Ly=4 * 3.141592653589793 / 3, Lz=4 * 3.141592653589793, Re=3000.0,
+                         Pr=0.72, Ma=1.5, viscexp=0.7, stretch=1.8, check_cfl=10, check_bulk=10)
+            o.set_state(*channel_fields(o.grid(), o.cfg, seed=1))
+        return o
+
+    pts = {"C1": 64 ** 3, "C2": 256 ** 3, "C3": 160 * 128 * 192}
+
+    def run(name, threads, steps, reps):
+        O.set_threads(threads)
+        o = make(name)
+        o.step(1)  # warm-up
+        times = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            o.step(steps)
+            times.append(time.perf_counter() - t0)
+        o.close()
+        t = sorted(times)[len(times) // 2]
+        return {"config": name, "threads": threads, "steps": steps, "repetitions": reps, "median_s": round(t, 4),
+                "gpt_rksteps_per_s": round(pts[name] * steps / t / 1e9, 6)}
+
+    runs = [run("C2", allc, 1, 3), run("C1", allc, 100, 5), run("C3", allc, 1, 3),
+            run("C1", 1, 10, 1), run("C3", 1, 1, 1)]
+    if full_c2_single:
+        runs.append(run("C2", 1, 1, 1))
+    head = runs[0]
+    return {"value": head["gpt_rksteps_per_s"], "unit": "Gpt-RKsteps/s", "cores": allc, "kind": "port",
+            "sample": f"256^3 order-8 Taylor-Green RK4: median of 3 single steps after one warm-up step on "
+                      f"{allc} physical cores (the FP64 oracle, OpenMP)",
+            "cpu_model": topo["model"], "cpus_allowed": topo["cpus_allowed"], "physical_cores": allc,
+            "runs": runs}
 
 
 def other_configs(cns, steps):
@@ -160,7 +229,8 @@ def ensemble_measure(cns, steps):
     seq = work / (seq_ms / 1e3) / 1e9
     bat = work / (bat_ms / 1e3) / 1e9
     return {"workload": "64 Taylor-Green instances, sizes {32,48,64,96,128}^3 (seed 4), order 6, RK4, "
-                        f"FP32 storage, {steps} steps each; sequential: each alone, launch shape tuned",
+                        f"FP32 storage, {steps} steps each; sequential: each alone, launch shape tuned, timed with "
+                        "cns_time_steps (bulk statistics every 10 steps included, which the batch does not compute)",
             "sequential_gpt_steps_per_s": round(seq, 4), "batched_gpt_steps_per_s": round(bat, 4),
             "speedup": round(bat / seq, 3), "sequential_ms": round(seq_ms, 2), "batched_ms": round(bat_ms, 2),
             "tune_s": round(tune_s, 3), "groups": groups}
@@ -174,8 +244,9 @@ def serve_measure(n_requests):
     from cudanavierstokes_amd.workloads import serving
     reqs = serving(n_requests=n_requests, seed=5, rate=20.0)
     _, summary = Server(budget_bytes=200e9, tick_steps=10).run(reqs, realtime=True)
-    summary["workload"] = (f"first {n_requests} of 1000 seeded requests (Poisson 20/s), 70% Taylor-Green "
-                           "48^3-160^3 / 30% channel, order 6, RK3, FP32 storage, 200 GB admission budget")
+    summary["workload"] = (f"{'all' if n_requests >= 1000 else f'first {n_requests} of'} 1000 seeded requests "
+                           "(Poisson 20/s), 70% Taylor-Green 48^3-160^3 / 30% channel, order 6, RK3, FP32 storage, "
+                           "200 GB admission budget, on the wall clock")
     return summary
 
 
@@ -264,6 +335,8 @@ def main():
                    "Re": 1600, "Ma": 0.1},
         "roofline": {"bound": "hbm", "achieved": round(m64["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(m64["achieved"] / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE per k_rhs launch, scaled by a copy "
+                                       "kernel, from profiles/traffic_k_rhs_tgv256_o8_rk4_fp64.txt (not this run)",
                      "kernel": "k_rhs (fused RHS + RK4 stage update)",
                      "kernel_ms_per_launch": round(m64["per_launch_ms"], 4), "launches": m64["launches"],
                      "pencil": m64["pencil"]},
@@ -290,7 +363,7 @@ def main():
     if a.other_steps > 0:
         out["other_configs"] = other_configs(cns, a.other_steps)
     if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(n, order, a.cpu_threads or host_cores())
+        out["cpu_baseline"] = cpu_baseline(a.cpu_c2_single)
     print(json.dumps(out), flush=True)
 
 

@@ -1287,26 +1287,45 @@ extern "C" int cns_tune(cns_solver *s, int *pencil) {
     return CNS_OK;
 }
 
+// Events of one timed region, destroyed on every return path.
+namespace {
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    ~EventPair() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+}  // namespace
+
+// Times nsteps steps exactly as cns_step queues them with all outputs: CFL
+// and forcing checks, the bulk statistics every check_bulk steps
+// (src/cuda_main.cu:109-186) and the time array; only the copies of those
+// small arrays to the host are left out.
 extern "C" int cns_time_steps(cns_solver *s, int nsteps, double *ms) {
     if (!s || !ms) return fail(s, CNS_ERR_ARG, "cns_time_steps: null argument");
+    if (nsteps < 1) return fail(s, CNS_ERR_ARG, "cns_time_steps: nsteps < 1");
     if (!s->have_state) return fail(s, CNS_ERR_STATE, "cns_time_steps: no state set");
     if (s->in_batch) return fail(s, CNS_ERR_STATE, "cns_time_steps: the solver is advanced by a batch");
     s->bsteps = 0;
     HIPCHK(s, hipSetDevice(s->dev));
-    hipEvent_t a, b;
-    HIPCHK(s, hipEventCreate(&a));
-    HIPCHK(s, hipEventCreate(&b));
+    int rc;
+    if ((rc = ensure_par(s, nsteps))) return rc;
+    EventPair ev;
+    HIPCHK(s, hipEventCreate(&ev.a));
+    HIPCHK(s, hipEventCreate(&ev.b));
     HIPCHK(s, hipStreamSynchronize(s->stream));
-    HIPCHK(s, hipEventRecord(a, s->stream));
-    int rc = queue_steps(s, nsteps, false, false);
-    if (rc) return rc;
-    HIPCHK(s, hipEventRecord(b, s->stream));
-    HIPCHK(s, hipEventSynchronize(b));
+    HIPCHK(s, hipEventRecord(ev.a, s->stream));
+    const double nan = std::nan("");
+    hipLaunchKernelGGL(k_fill, dim3(64), dim3(256), 0, s->stream, s->d_par1, (long)nsteps, nan);
+    hipLaunchKernelGGL(k_fill, dim3(64), dim3(256), 0, s->stream, s->d_par2, (long)nsteps, nan);
+    LAUNCHED(s, "k_fill");
+    if ((rc = queue_steps(s, nsteps, true, true))) return rc;
+    HIPCHK(s, hipEventRecord(ev.b, s->stream));
+    HIPCHK(s, hipEventSynchronize(ev.b));
     float f = 0;
-    HIPCHK(s, hipEventElapsedTime(&f, a, b));
+    HIPCHK(s, hipEventElapsedTime(&f, ev.a, ev.b));
     *ms = f;
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
     return CNS_OK;
 }
 

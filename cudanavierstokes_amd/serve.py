@@ -6,9 +6,10 @@ footprint of the admitted states stays within a budget; admitted states stay
 resident until their last step.  Every tick advances all admitted requests
 together with one batch (cns_batch_*: one launch per RK stage per kernel
 group, one replayed graph per step), by the smallest remaining step count or
-the tick length, whichever is less; finished requests leave (their results
-are read back while the next tick runs), waiting ones are admitted, and the
-batch is rebuilt for the new membership.  Each request's
+the tick length, whichever is less; finished requests leave (a worker
+thread reads their results while the next ticks run, and their device
+memory is freed between ticks), waiting ones are admitted, and the batch is
+rebuilt for the new membership.  Each request's
 result equals its lone cns_step result to rounding (the batch keeps every
 member's step count, so the CFL checks fall on the same steps).
 
@@ -28,6 +29,7 @@ import collections
 import json
 import os
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -101,74 +103,93 @@ class Server:
                 joining[r["id"]] = dict(req=r, solver=s, left=int(r["steps"]), admitted=clock(), bytes=need)
                 used[0] += need
 
-        finishing = []  # done on the GPU; results are read while the next tick runs
+        # Finished requests: their results are read (cns_get_state on their
+        # own streams) by a worker thread while the next ticks run; their
+        # solvers are closed on this thread between ticks, when the GPU is
+        # idle anyway (freeing device memory waits for the whole device).
+        reader = ThreadPoolExecutor(max_workers=2)
+        reading = []  # (future, entry)
 
-        def finish():
+        def read_result(a):
+            r = a["req"]
+            state = a["solver"].get_state()
+            t_done = clock()
+            rec = dict(id=r["id"], kind=r["kind"], arrival=r["arrival"], admitted=a["admitted"],
+                       done=t_done, tts=t_done - r["arrival"], steps=int(r["steps"]),
+                       points=points(r["config"]))
+            if self.keep:
+                rec["state"] = state
+            if self.ckpt is not None:
+                ts = int(r["steps"]) + (int(r["restart"]["timestep"]) if "restart" in r else 0)
+                d = os.path.join(self.ckpt, str(r["id"]))
+                save_checkpoint(a["solver"], d, ts)
+                rec["checkpoint"] = dict(dir=d, timestep=ts)
+            return rec
+
+        def release(block):
+            """Close the solvers whose results are read (all of them with
+            block); frees their share of the budget."""
             tf = time.perf_counter()
-            for a in finishing:
-                r = a["req"]
-                state = a["solver"].get_state()
-                t_done = clock()
-                rec = dict(id=r["id"], kind=r["kind"], arrival=r["arrival"], admitted=a["admitted"],
-                           done=t_done, tts=t_done - r["arrival"], steps=int(r["steps"]),
-                           points=points(r["config"]))
-                if self.keep:
-                    rec["state"] = state
-                if self.ckpt is not None:
-                    ts = int(r["steps"]) + (int(r["restart"]["timestep"]) if "restart" in r else 0)
-                    d = os.path.join(self.ckpt, str(r["id"]))
-                    save_checkpoint(a["solver"], d, ts)
-                    rec["checkpoint"] = dict(dir=d, timestep=ts)
-                done.append(rec)
-                a["solver"].close()
-                used[0] -= a["bytes"]
-                if log:
-                    log(rec)
-            finishing.clear()
-            cost["finish_s"] += time.perf_counter() - tf
+            keep = []
+            for fut, a in reading:
+                if block or fut.done():
+                    rec = fut.result()
+                    done.append(rec)
+                    a["solver"].close()
+                    used[0] -= a["bytes"]
+                    if log:
+                        log(rec)
+                else:
+                    keep.append((fut, a))
+            reading[:] = keep
+            cost["release_s"] += time.perf_counter() - tf
 
-        while pending or waiting or active or joining or finishing:
-            admit()
-            if joining:
-                active.update(joining)
-                joining.clear()
-                if batch is not None:
+        finished_now = []
+        try:
+            while pending or waiting or active or joining or reading or finished_now:
+                for a in finished_now:
+                    reading.append((reader.submit(read_result, a), a))
+                finished_now = []
+                admit()
+                if joining:
+                    active.update(joining)
+                    joining.clear()
+                    if batch is not None:
+                        batch.close()
+                        batch = None
+                if not active:
+                    if reading:
+                        release(True)  # frees memory: admit again before waiting
+                        continue
+                    if pending:
+                        wait = pending[0]["arrival"] - clock()
+                        if wait > 0:
+                            if realtime:
+                                time.sleep(wait)
+                            else:
+                                shift[0] += wait
+                    continue
+                if batch is None:
+                    tb = time.perf_counter()
+                    batch = Batch([a["solver"] for a in active.values()], tune=self.tune)
+                    cost["batch_build_s"] += time.perf_counter() - tb
+                k = min(self.tick, min(a["left"] for a in active.values()))
+                tb = time.perf_counter()
+                batch.launch(k)
+                admit()  # newcomers' state upload overlaps the tick
+                batch.wait()
+                busy += time.perf_counter() - tb
+                ticks += 1
+                release(False)
+                finished = [rid for rid, a in active.items() if a["left"] == k]
+                for a in active.values():
+                    a["left"] -= k
+                if finished:
                     batch.close()
                     batch = None
-            if not active:
-                if finishing:
-                    finish()  # frees memory: admit again before waiting
-                    continue
-                if pending:
-                    wait = pending[0]["arrival"] - clock()
-                    if wait > 0:
-                        if realtime:
-                            time.sleep(wait)
-                        else:
-                            shift[0] += wait
-                continue
-            if batch is None:
-                tb = time.perf_counter()
-                batch = Batch([a["solver"] for a in active.values()], tune=self.tune)
-                cost["batch_build_s"] += time.perf_counter() - tb
-            k = min(self.tick, min(a["left"] for a in active.values()))
-            tb = time.perf_counter()
-            batch.launch(k)
-            # host work overlaps the batch on the GPU: results of the requests
-            # that finished at the last tick, then newcomers
-            finish()
-            admit()
-            batch.wait()
-            busy += time.perf_counter() - tb
-            ticks += 1
-            finished = [rid for rid, a in active.items() if a["left"] == k]
-            for a in active.values():
-                a["left"] -= k
-            if finished:
-                batch.close()
-                batch = None
-                for rid in finished:
-                    finishing.append(active.pop(rid))
+                    finished_now = [active.pop(rid) for rid in finished]
+        finally:
+            reader.shutdown(wait=True)
         if batch is not None:
             batch.close()
         wall = clock()

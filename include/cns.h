@@ -229,8 +229,11 @@ int cns_sync(cns_solver *s);
 int cns_profile_enable(cns_solver *s, int enable);
 int cns_profile_read(cns_solver *s, double *total_ms, int *launches);
 
-/* Steps with inputs resident on the device, timed with HIP events on the
- * solver's stream; *ms receives the elapsed time of the n_steps. */
+/* n_steps steps exactly as cns_step queues them with all outputs (CFL and
+ * forcing checks, the bulk statistics every check_bulk steps, the time
+ * array), inputs resident on the device, timed with HIP events on the
+ * solver's stream; *ms receives the elapsed time.  Only the copies of the
+ * per-step arrays to the host are left out. */
 int cns_time_steps(cns_solver *s, int n_steps, double *ms);
 
 /* Launch shape of the stage kernels: each workgroup marches a pencil of
