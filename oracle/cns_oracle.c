@@ -17,7 +17,9 @@
  *   rhs_y           <- src/cuda_rhs.cu:130-262             (deviceRHSY, adds +=)
  *   rhs_z           <- src/cuda_rhs.cu:264-396             (deviceRHSZ, adds +=, forcing)
  *   add_sponge      <- src/sponge.cu:31-41                 (addSponge)
- *   ghost fills     <- src/boundary.h:38-148, src/boundary_condition_{x,y,z}.h
+ *   ghost fills     <- src/boundary.h:38-180, src/boundary_condition_{x,y,z}.h
+ *                      (periodic, isothermal wall, boundary layer)
+ *   blowing         <- src/perturbation.h:25-53                (PerturbUvel)
  *   stencils/fluxes <- src/cuda_derivs.h:30-312
  *   SMR RK3         <- src/cuda_main.cu:10-11,109-186,244-247 (race-free: the
  *                      momentum is formed with the density before the update)
@@ -33,7 +35,10 @@
  * wavenumbers, polynomial exactness, split-flux identities, free stream,
  * discrete conservation, manufactured-solution convergence of the full RHS
  * against a symbolic continuous RHS (periodic and isothermal-wall cases),
- * RK stability polynomials, dt and reduction closed forms.
+ * RK stability polynomials, dt and reduction closed forms, boundary-layer
+ * ghost rules (exact derivatives of linear fields at the extrapolated ends,
+ * adiabatic rest state, pressure extrapolated on its own) and the blowing's
+ * band, zeros, antisymmetry, spanwise cosine and hand-evaluated values.
  * Unpinned: libm pow/tanh last-ulp behaviour against CUDA libdevice.
  */
 #include <math.h>
@@ -58,6 +63,8 @@ typedef struct orc_config {
     int check_cfl;
     int check_bulk;
     int fix_y_work;     /* 0: reference viscous-work term in y; 1: corrected */
+    int boundary_layer; /* 1 (with periodic_x 0): boundary-layer ghost rules in x and z */
+    int perturbed;      /* 1: wall-normal blowing at the bottom x boundary (PerturbUvel) */
     double Lx, Ly, Lz;
     double Re, Pr, Ma, gam, viscexp, cfl;
     double stretch, twall_bot, twall_top;
@@ -151,10 +158,14 @@ static void init_grid(orc *o) {
     const orc_config *c = &o->c;
     int mx = c->mx, my = c->my, mz = c->mz;
     o->dx = c->Lx * (1.0) / (mx);
+    /* a boundary layer takes the lower half of the tanh profile (clustered at
+       the wall x = 0 only): src/init.cpp:41-50 */
+    int denom = c->boundary_layer ? 2 * mx : mx;
+    double denom2 = c->boundary_layer ? 1.0 : 2.0;
     double *xn = (double *)malloc(sizeof(double) * (mx + 1));
     for (int i = 0; i < mx + 1; i++)
-        xn[i] = tanh(c->stretch * ((i * 1.0) / mx - 0.5)) / tanh(c->stretch * 0.5);
-    for (int i = 0; i < mx; i++) o->x[i] = c->Lx * (1.0 + (xn[i] + xn[i + 1]) / 2.0) / 2.0;
+        xn[i] = tanh(c->stretch * ((i * 1.0) / denom - 0.5)) / tanh(c->stretch * 0.5);
+    for (int i = 0; i < mx; i++) o->x[i] = c->Lx * (1.0 + (xn[i] + xn[i + 1]) / 2.0) / denom2;
     free(xn);
     deriv_grid(o, o->xpp, o->xp, o->x, o->dx);
     for (int i = 0; i < mx; i++) o->xp[i] = 1.0 / o->xp[i];
@@ -199,6 +210,7 @@ int orc_config_size(void) { return (int)sizeof(orc_config); }
 orc *orc_create(const orc_config *cfg) {
     if (cfg->s < 1 || cfg->s > 4 || cfg->sv < 1 || cfg->sv > cfg->s) return NULL;
     if (cfg->mx < 2 * cfg->s || cfg->my < 2 * cfg->s || cfg->mz < 2 * cfg->s) return NULL;
+    if (cfg->boundary_layer && cfg->periodic_x) return NULL;
     orc *o = (orc *)calloc(1, sizeof(orc));
     o->c = *cfg;
     o->S = cfg->s;
@@ -381,6 +393,61 @@ static inline void g_ext(double *b, int n, int S, double top, double bot) { /* w
     }
 }
 
+/* boundary layer (src/boundary.h:150-180): linear extrapolation about the
+   last node (topBCxExt, topBCzExt) and about the first node (botBCzExt);
+   at the bottom x boundary the odd and even reflections about the face are
+   g_vel's and g_mir's bottom halves (botBCxExt with value 0, botBCxMir) */
+static inline void g_ext_top(double *b, int n, int S) {
+    for (int d = 1; d <= S; d++) b[S + n - 1 + d] = 2.0 * b[S + n - 1] - b[S + n - 1 - d];
+}
+static inline void g_ext_bot(double *b, int S) {
+    for (int d = 1; d <= S; d++) b[S - d] = 2.0 * b[S] - b[S + d];
+}
+static inline void g_odd_bot(double *b, int S) {
+    for (int l = 0; l < S; l++) b[l] = -b[2 * S - l - 1];
+}
+static inline void g_mir_bot(double *b, int S) {
+    for (int l = 0; l < S; l++) b[l] = b[2 * S - l - 1];
+}
+
+/* Wall-normal blowing of src/perturbation.h:25-53 (PerturbUvel): for global
+   z index k in [kC - LP/2, kC + LP/2] every bottom ghost of u takes
+   kappa g(xi) [amp1 sin(omega1 t) + amp2 sin(omega2 t) cos(y/lambda)] with
+   g = 15.1875 xi^5 - 35.4375 xi^4 + 20.25 xi^3, kappa = +1 before kC and -1
+   from kC on, xi the integer distance from the nearer end over LP/2,
+   y = j/d_dy, lambda = Ly/(2 pi), omega1 = Re 121e-6, omega2 = 10, t the
+   simulation time after the step's advance.  Returns 1 and the value when
+   k is in the band. */
+int orc_perturbation(const orc *o, int j, int k, double t, double *val) {
+    const int kC = 110, LP = 40;
+    const int kSt = kC - LP / 2, kEn = kC + LP / 2;
+    if (k < kSt || k > kEn) return 0;
+    int kappa, alpha, beta;
+    if (k < kC) {
+        kappa = 1;
+        alpha = k - kSt;
+        beta = kC - kSt;
+    } else {
+        kappa = -1;
+        alpha = kEn - k;
+        beta = kEn - kC;
+    }
+    double xi = alpha * 1.0 / beta;
+    double g = (15.1875 * xi * xi * xi * xi * xi) - (35.4375 * xi * xi * xi * xi) + (20.25 * xi * xi * xi);
+    double lam = o->c.Ly / (2.0 * M_PI);
+    double om1 = o->c.Re * 121.e-6, om2 = 10.0;
+    double y = j / o->d_dy;
+    *val = 2e-4 * kappa * g * sin(om1 * t) + 3e-5 * kappa * g * sin(om2 * t) * cos(y / lam);
+    return 1;
+}
+
+/* u's bottom ghosts at (j, k) when the blowing applies */
+static inline void g_perturb(const orc *o, double *b, int S, int j, int k) {
+    double val;
+    if (o->c.boundary_layer && o->c.perturbed && orc_perturbation(o, j, k, o->time, &val))
+        for (int l = 0; l < S; l++) b[l] = val;
+}
+
 /* ------------------------------------------------------------------ */
 /* velocity gradients (src/calc_stress.cu:20-63)                       */
 /* ------------------------------------------------------------------ */
@@ -400,7 +467,15 @@ static void der_vel(orc *o) {
                 for (int q = 0; q < 3; q++) {
                     const double *f = fld[q];
                     for (int i = 0; i < mx; i++) b[i + S] = f[i + j * mx + k * mxy];
-                    if (c->periodic_x) g_per(b, mx, S); else g_vel(b, mx, S);
+                    if (c->periodic_x) {
+                        g_per(b, mx, S);
+                    } else if (c->boundary_layer) { /* BCxderVel, src/boundary_condition_x.h:22-31 */
+                        g_ext_top(b, mx, S);
+                        g_odd_bot(b, S);
+                        if (q == 0) g_perturb(o, b, S, j, k);
+                    } else {
+                        g_vel(b, mx, S);
+                    }
                     for (int i = 0; i < mx; i++) o->g[q][i + j * mx + k * mxy] = dv1x(o, b, i + S, i);
                 }
             }
@@ -423,7 +498,12 @@ static void der_vel(orc *o) {
                 for (int q = 0; q < 3; q++) {
                     const double *f = fld[q];
                     for (int k = 0; k < mz; k++) b[k + SV] = f[i + j * mx + k * mxy];
-                    g_per(b, mz, SV);
+                    if (c->boundary_layer) { /* BCzderVel, src/boundary_condition_z.h:28-34 */
+                        g_ext_top(b, mz, SV);
+                        g_ext_bot(b, SV);
+                    } else {
+                        g_per(b, mz, SV);
+                    }
                     for (int k = 0; k < mz; k++)
                         o->g[6 + q][i + j * mx + k * mxy] = d1g(o->cVF, SV, o->d_dz, b, k + SV);
                 }
@@ -466,6 +546,21 @@ static void rhs_x(orc *o, double **out) {
                 if (c->periodic_x) {
                     g_per(su, mx, S); g_per(sv_, mx, S); g_per(sw, mx, S); g_per(st, mx, S);
                     g_per(sp, mx, S); g_per(sm, mx, S); g_per(sl, mx, S);
+                } else if (c->boundary_layer) { /* src/boundary_condition_x.h:60-74 */
+                    g_ext_top(su, mx, S); g_ext_top(sv_, mx, S); g_ext_top(sw, mx, S);
+                    g_ext_top(sp, mx, S); g_ext_top(st, mx, S);
+                    g_mir_bot(sp, S); g_mir_bot(st, S);
+                    g_odd_bot(su, S); g_odd_bot(sv_, S); g_odd_bot(sw, S);
+                    g_perturb(o, su, S, j, k);
+                    for (int l = 0; l < S; l++) {
+                        int gi[2] = {l, l + S + mx};
+                        for (int q = 0; q < 2; q++) {
+                            int ix = gi[q];
+                            double suth = pow(st[ix], c->viscexp);
+                            sm[ix] = suth / c->Re;
+                            sl[ix] = suth / c->Re / c->Pr / o->Ec;
+                        }
+                    }
                 } else {
                     g_mir(sp, mx, S);
                     g_vel(su, mx, S); g_vel(sv_, mx, S); g_vel(sw, mx, S);
@@ -482,7 +577,14 @@ static void rhs_x(orc *o, double **out) {
                     }
                 }
                 /* BCxNumber2 */
-                if (c->periodic_x) g_per(sd, mx, S); else g_mir(sd, mx, S);
+                if (c->periodic_x) {
+                    g_per(sd, mx, S);
+                } else if (c->boundary_layer) {
+                    g_ext_top(sd, mx, S);
+                    g_mir_bot(sd, S);
+                } else {
+                    g_mir(sd, mx, S);
+                }
                 /* BCxNumber3 */
                 if (c->periodic_x) {
                     g_per(sr, mx, S); g_per(sh, mx, S);
@@ -565,8 +667,18 @@ static void rhs_yz(orc *o, double **out, int dir) {
                     sm[q + S] = o->m[g]; sd[q + S] = o->dil[g]; sp[q + S] = o->p[g];
                     sl[q + S] = o->l[g]; st[q + S] = o->t[g]; sr[q + S] = o->r[g]; sh[q + S] = o->h[g];
                 }
-                g_per(su, n, S); g_per(sv_, n, S); g_per(sw, n, S); g_per(sm, n, S); g_per(sd, n, S);
-                g_per(sp, n, S); g_per(sl, n, S); g_per(st, n, S); g_per(sr, n, S); g_per(sh, n, S);
+                if (dir == 2 && c->boundary_layer) {
+                    /* BCzNumber1-4 (src/boundary_condition_z.h): each staged
+                       quantity extrapolated on its own about the end nodes */
+                    double *all[10] = {su, sv_, sw, sm, sd, sp, sl, st, sr, sh};
+                    for (int q = 0; q < 10; q++) {
+                        g_ext_top(all[q], n, S);
+                        g_ext_bot(all[q], S);
+                    }
+                } else {
+                    g_per(su, n, S); g_per(sv_, n, S); g_per(sw, n, S); g_per(sm, n, S); g_per(sd, n, S);
+                    g_per(sp, n, S); g_per(sl, n, S); g_per(st, n, S); g_per(sr, n, S); g_per(sh, n, S);
+                }
                 /* the convecting velocity and the diagonal component */
                 const double *sa = (dir == 1) ? sv_ : sw;
                 for (int q = 0; q < n; q++) {
@@ -853,6 +965,7 @@ void orc_set_sponge(orc *o, const double *sx, const double *sz, const double *rr
 double orc_get_dt(const orc *o) { return o->dt; }
 double orc_get_dpdz(const orc *o) { return o->dpdz; }
 double orc_get_time(const orc *o) { return o->time; }
+void orc_set_time(orc *o, double t) { o->time = t; }
 void orc_set_dt(orc *o, double dt) { o->dt = dt; }
 double orc_calc_dt(const orc *o) { return calc_dt(o); }
 void orc_bulk(orc *o, double *par1, double *par2) { calc_bulk(o, par1, par2); }

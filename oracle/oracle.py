@@ -21,6 +21,7 @@ class OrcConfig(C.Structure):
         ("periodic_x", C.c_int), ("nonuniform_x", C.c_int),
         ("forcing", C.c_int), ("sponge", C.c_int), ("scheme", C.c_int),
         ("check_cfl", C.c_int), ("check_bulk", C.c_int), ("fix_y_work", C.c_int),
+        ("boundary_layer", C.c_int), ("perturbed", C.c_int),
         ("Lx", C.c_double), ("Ly", C.c_double), ("Lz", C.c_double),
         ("Re", C.c_double), ("Pr", C.c_double), ("Ma", C.c_double),
         ("gam", C.c_double), ("viscexp", C.c_double), ("cfl", C.c_double),
@@ -60,6 +61,9 @@ def lib():
             getattr(L, f).restype = C.c_double
             getattr(L, f).argtypes = [vp]
         L.orc_set_dt.argtypes = [vp, C.c_double]
+        L.orc_set_time.argtypes = [vp, C.c_double]
+        L.orc_perturbation.restype = C.c_int
+        L.orc_perturbation.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.POINTER(C.c_double)]
         L.orc_bulk.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.orc_vol_integral.restype = C.c_double
         L.orc_vol_integral.argtypes = [vp, dp, C.c_int]
@@ -81,7 +85,7 @@ def lib():
 def make_config(**kw):
     """OrcConfig from keyword arguments (same names as the struct)."""
     d = dict(mx=16, my=16, mz=16, s=2, sv=2, periodic_x=1, nonuniform_x=0, forcing=0,
-             sponge=0, scheme=0, check_cfl=10, check_bulk=10, fix_y_work=0,
+             sponge=0, scheme=0, check_cfl=10, check_bulk=10, fix_y_work=0, boundary_layer=0, perturbed=0,
              Lx=2 * np.pi, Ly=2 * np.pi, Lz=2 * np.pi, Re=1600.0, Pr=0.71, Ma=0.1,
              gam=1.4, viscexp=1.0, cfl=0.5, stretch=1.8, twall_bot=1.0, twall_top=1.0,
              dpdz0=0.00372)
@@ -175,6 +179,16 @@ class Oracle:
     @property
     def time(self):
         return self.L.orc_get_time(self.h)
+
+    @time.setter
+    def time(self, t):
+        self.L.orc_set_time(self.h, float(t))
+
+    def perturbation(self, j, k, t):
+        """Blowing value of u's bottom ghosts at (j, k) and time t, or None
+        outside the band."""
+        v = C.c_double()
+        return v.value if self.L.orc_perturbation(self.h, int(j), int(k), float(t), C.byref(v)) else None
 
     def calc_dt(self):
         return self.L.orc_calc_dt(self.h)
