@@ -29,6 +29,7 @@ class CnsConfig(C.Structure):
         ("reynolds", C.c_double), ("prandtl", C.c_double), ("mach", C.c_double), ("gamma", C.c_double),
         ("visc_exp", C.c_double), ("cfl", C.c_double), ("stretch", C.c_double),
         ("twall_bot", C.c_double), ("twall_top", C.c_double), ("dpdz0", C.c_double),
+        ("boundary_layer", C.c_int), ("perturbed", C.c_int),
     ]
 
 

@@ -1,7 +1,8 @@
 """Build the gfx950 shared library libcns.so in-tree with hipcc.
 
 The kernels are compiled once per advective stencil half width (CNS_S=1..4)
-in parallel; the host API is its own translation unit.  Packed FP32 vector
+and storage type in parallel, the boundary-layer forms (FP64) once per half
+width; the host API is its own translation unit.  Packed FP32 vector
 instructions are disabled for all device code.
 """
 import concurrent.futures as cf
@@ -39,6 +40,8 @@ def build(verbose=False, jobs=None):
         for r in ("double", "float"):
             units.append((os.path.join(CSRC, "cns_kernels.hip"), os.path.join(BUILD, f"cns_kernels_s{s}_{r}.o"),
                           [f"-DCNS_S={s}", f"-DCNS_R={r}"]))
+        units.append((os.path.join(CSRC, "cns_kernels_bl.hip"), os.path.join(BUILD, f"cns_kernels_bl_s{s}.o"),
+                      [f"-DCNS_S={s}"]))
     units.append((os.path.join(CSRC, "cns_api.hip"), os.path.join(BUILD, "cns_api.o"), []))
     cmds = [[HIPCC] + CFLAGS + extra + ["-c", src, "-o", obj] for src, obj, extra in units]
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:

@@ -24,6 +24,7 @@
 #include <cstring>
 #include <string>
 #include <thread>
+#include <type_traits>
 #include <mutex>
 #include <vector>
 
@@ -226,9 +227,13 @@ static void build_grid(cns_solver *s) {
     s->y.assign(my, 0.0);
     s->z.assign(mz, 0.0);
     s->dx = c.len_x * (1.0) / (mx);
+    // a boundary layer clusters x at the wall x = 0 only: the lower half of
+    // the tanh profile (src/init.cpp:41-50)
+    const int denom = c.boundary_layer ? 2 * mx : mx;
+    const double denom2 = c.boundary_layer ? 1.0 : 2.0;
     std::vector<double> xn(mx + 1);
-    for (int i = 0; i < mx + 1; i++) xn[i] = tanh(c.stretch * ((i * 1.0) / mx - 0.5)) / tanh(c.stretch * 0.5);
-    for (int i = 0; i < mx; i++) s->x[i] = c.len_x * (1.0 + (xn[i] + xn[i + 1]) / 2.0) / 2.0;
+    for (int i = 0; i < mx + 1; i++) xn[i] = tanh(c.stretch * ((i * 1.0) / denom - 0.5)) / tanh(c.stretch * 0.5);
+    for (int i = 0; i < mx; i++) s->x[i] = c.len_x * (1.0 + (xn[i] + xn[i + 1]) / 2.0) / denom2;
     // derivatives of x(xi) with odd/2Lx-odd mirrored ghosts, advective order
     std::vector<double> fb(mx + 2 * S);
     for (int i = S; i < mx + S; i++) fb[i] = s->x[i - S];
@@ -282,7 +287,15 @@ static void build_grid(cns_solver *s) {
 // grids want chunks of 8.  cns_tune measures instead.
 static void set_pencil(cns_solver *s, int zc) {
     const cns_config &c = s->cfg;
-    s->zc = std::max(1, std::min(zc, c.nz));
+    zc = std::max(1, std::min(zc, c.nz));
+    if (c.boundary_layer) {
+        // every chunk at least S planes long: no window start of the RHS
+        // kernel reaches past a z end (the kernels extrapolate only planes
+        // they march over)
+        zc = std::max(zc, s->S);
+        while (c.nz % zc != 0 && c.nz % zc < s->S) zc++;
+    }
+    s->zc = zc;
     s->nzc = (c.nz + s->zc - 1) / s->zc;
     s->nblocks = s->ntx * s->nty * s->nzc;
     s->base.zc = s->zc;
@@ -346,6 +359,10 @@ static void fill_params(cns_solver *s) {
     P.cpR = s->Rgas * c.gamma / (c.gamma - 1.0);
     P.twall_bot = c.twall_bot;
     P.twall_top = c.twall_top;
+    P.blayer = c.boundary_layer ? 1 : 0;
+    P.perturbed = (c.boundary_layer && c.perturbed) ? 1 : 0;
+    P.blow_om1 = c.reynolds * 121.e-6;
+    P.blow_lam = c.len_y / (2.0 * M_PI);
     for (int i = 0; i < 4; i++) {
         P.cF[i] = s->cF[i];
         P.cVF[i] = s->cVF[i];
@@ -362,6 +379,7 @@ static void fill_params(cns_solver *s) {
     P.ref = s->d_ref;
     P.dt = &s->d_sc->dt;
     P.dpdz = &s->d_sc->dpdz;
+    P.time = &s->d_sc->time;
     P.visc_max_out = &s->d_sc->visc_max;
     P.comp = s->comp ? 1 : 0;
 }
@@ -394,6 +412,8 @@ extern "C" int cns_config_init(cns_config *c) {
     c->twall_bot = 1.0;
     c->twall_top = 1.0;
     c->dpdz0 = 0.00372;
+    c->boundary_layer = 0;
+    c->perturbed = 0;
     return CNS_OK;
 }
 
@@ -438,6 +458,25 @@ static int validate(const cns_config *c, std::string &msg) {
     if (c->precision != CNS_FP64 && c->precision != CNS_FP32) {
         msg = "precision must be CNS_FP64 or CNS_FP32";
         return CNS_ERR_ARG;
+    }
+    if (c->boundary_layer) {
+        if (c->periodic_x) {
+            msg = "boundary_layer needs periodic_x = 0";
+            return CNS_ERR_ARG;
+        }
+        if (c->precision != CNS_FP64) {
+            msg = "boundary_layer runs with FP64 storage only";
+            return CNS_ERR_ARG;
+        }
+        if (c->nx % 16 != 0) {
+            // the top x ghosts are formed from the last 16-wide tile's nodes
+            msg = "boundary_layer needs nx to be a multiple of 16";
+            return CNS_ERR_ARG;
+        }
+        if (c->nz <= 2 * c->stencil_size) {
+            msg = "boundary_layer needs nz > 2*stencil_size";
+            return CNS_ERR_ARG;
+        }
     }
     return CNS_OK;
 }
@@ -873,6 +912,36 @@ static int run_stage_on(cns_solver *s, const R *qin, R *dil, Params P, StagePtrs
     A.fs = s->n;
     A.dil = dil;
     fill_unused(A);
+    if constexpr (std::is_same<R, double>::value) {
+        if (s->cfg.boundary_layer) {
+            // boundary layer: both kernels on the RHS kernel's 16 x 16 tiles
+            // and z chunks (the last x tile holds the nodes the top ghosts
+            // extrapolate; every chunk is at least S planes long)
+            HIPCHK(s, launch_dil_bl(s->SV, nonuni, s->nblocks, s->stream, P, A, dil));
+            LAUNCHED(s, "k_dil");
+            if (P.want_visc_max) {
+                hipLaunchKernelGGL(k_reset_visc, dim3(1), dim3(1), 0, s->stream, s->d_sc);
+                LAUNCHED(s, "k_reset_visc");
+            }
+            const bool prof = s->prof && timing;
+            if (prof) {
+                const size_t need = 2 * (size_t)(s->prof_n + 1);
+                while (s->ev.size() < need) {
+                    hipEvent_t e;
+                    HIPCHK(s, hipEventCreate(&e));
+                    s->ev.push_back(e);
+                }
+                HIPCHK(s, hipEventRecord(s->ev[2 * s->prof_n], s->stream));
+            }
+            HIPCHK(s, launch_rhs_bl(s->S, s->SV, nonuni, s->nblocks, s->stream, P, A));
+            LAUNCHED(s, "k_rhs");
+            if (prof) {
+                HIPCHK(s, hipEventRecord(s->ev[2 * s->prof_n + 1], s->stream));
+                s->prof_n++;
+            }
+            return CNS_OK;
+        }
+    }
     // the dilatation kernel needs few registers, so it always runs 512-lane
     // workgroups on 32 x 16 tiles (fewer halo points per point), with its own
     // tile grid and z chunks: about 1024 workgroups, chunks of 8..the RHS's
@@ -2005,6 +2074,8 @@ extern "C" int cns_batch_create(cns_solver *const *members, int n, cns_batch **o
         if (!m->have_state) return bfail(nullptr, CNS_ERR_STATE, "cns_batch_create: member without a state");
         if (m->in_batch) return bfail(nullptr, CNS_ERR_STATE, "cns_batch_create: member already in a batch");
         if (m->dev != members[0]->dev) return bfail(nullptr, CNS_ERR_ARG, "cns_batch_create: members on different devices");
+        if (m->cfg.boundary_layer)
+            return bfail(nullptr, CNS_ERR_ARG, "cns_batch_create: boundary-layer solvers run alone, not in batches");
         for (int q = 0; q < k; q++)
             if (members[q] == m) return bfail(nullptr, CNS_ERR_ARG, "cns_batch_create: repeated member");
     }

@@ -43,6 +43,13 @@ struct Params {
     // FP32 storage, wall ghosts in offsets: wall temperature minus T0 at the
     // bottom and top walls, and cp T0 - h0 (zero up to rounding)
     double dtw_bot, dtw_top, ch0;
+    // boundary layer (FP64): extrapolated x top and z ends, adiabatic x
+    // bottom, optional wall-normal blowing at the bottom (omega1 = Re 121e-6,
+    // lambda = Ly / 2 pi; time is the simulation time after the step's
+    // advance, read on the device)
+    int blayer, perturbed;
+    double blow_om1, blow_lam;
+    const double *time;
     double cF[4], cS[5], cVF[4], cVS[5];
     const double *xp, *vsx, *dxv, *spx, *spz;
     const double *ref;  // 5 sponge reference fields of mx*mz, contiguous
@@ -193,6 +200,40 @@ __device__ __forceinline__ void wall_ghost(const Params &P, R tw, R &r, R &u, R 
     r = p / (R(P.Rgas) * t);
 }
 
+// Boundary layer, bottom x boundary (src/boundary_condition_x.h:60-74,
+// src/boundary.h:121-180): an adiabatic no-slip reflection about the face.
+// The ghost is the mirrored point with odd velocities and even p and T (so
+// T and mu are kept), u replaced by the blowing value ub when blow is set,
+// and rho and h formed from the ghost p, T and velocities.
+template <typename R>
+__device__ __forceinline__ void bl_bottom_ghost(const Params &P, bool blow, R ub, R &r, R &u, R &v, R &w, R t,
+                                                R &h) {
+    const R p = r * R(P.Rgas) * t;
+    u = blow ? ub : -u;
+    v = -v;
+    w = -w;
+    h = t * R(P.cpR) + R(0.5) * (u * u + v * v + w * w);
+    r = p / (R(P.Rgas) * t);
+}
+
+// Wall-normal blowing of src/perturbation.h:25-53 at global z index k: on
+// k in [90, 130] it is kappa g(xi) [2e-4 s1 + 3e-5 s2 cos(y/lambda)] with
+// g = 15.1875 xi^5 - 35.4375 xi^4 + 20.25 xi^3, kappa = +1 below k = 110 and
+// -1 from there on, xi the integer distance from the nearer end of the band
+// over 20; s1 = sin(omega1 t), s2 = sin(10 t) and cosj = cos(y/lambda) come
+// from the caller.  Returns whether k is in the band.
+__device__ __forceinline__ bool bl_blow(int k, double s1, double s2, double cosj, double &val) {
+    const int kC = 110, LP = 40, kSt = kC - LP / 2, kEn = kC + LP / 2;
+    if (k < kSt || k > kEn) return false;
+    const int kappa = k < kC ? 1 : -1;
+    const int alpha = k < kC ? k - kSt : kEn - k;
+    const int beta = k < kC ? kC - kSt : kEn - kC;
+    const double xi = alpha * 1.0 / beta;
+    const double g = (15.1875 * xi * xi * xi * xi * xi) - (35.4375 * xi * xi * xi * xi) + (20.25 * xi * xi * xi);
+    val = 2e-4 * kappa * g * s1 + 3e-5 * kappa * g * s2 * cosj;
+    return true;
+}
+
 // 1/x: correctly rounded division for FP64 storage (parity with the FP64
 // reference); the hardware reciprocal (1 ulp) for FP32 storage, whose
 // tolerance is 1e-5.
@@ -254,5 +295,8 @@ __device__ __forceinline__ void wall_ghost_off(const Params &P, R dtw, R &rp, R 
 // Quantity indices of the staged primitives (LDS plane and z register
 // window).  For mu = T/Re the last one is not staged: it is rebuilt from T.
 enum { QR = 0, QU, QV, QW, QH, QT, QD, QM, NQ };
+// pressure over nothing else: the boundary layer's z window carries p = rho R T
+// as its own quantity, since the reference extrapolates it on its own
+constexpr int QP = NQ;
 
 }  // namespace cns

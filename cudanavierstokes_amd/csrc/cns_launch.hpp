@@ -37,6 +37,40 @@ CNS_DECLARE_LAUNCH(3)
 CNS_DECLARE_LAUNCH(4)
 #undef CNS_DECLARE_LAUNCH
 
+// boundary-layer forms (FP64, one solver; cns_kernels_bl.hip)
+#define CNS_DECLARE_LAUNCH_BL(N)                                                                           \
+    hipError_t launch_rhs_bl_s##N(int sv, bool nonuni, int nblocks, hipStream_t st, const Params &P,      \
+                                  const StagePtrs<double> &A);                                           \
+    hipError_t launch_dil_bl_s##N(bool nonuni, int nblocks, hipStream_t st, const Params &P,              \
+                                  const StagePtrs<double> &A, double *dil);
+CNS_DECLARE_LAUNCH_BL(1)
+CNS_DECLARE_LAUNCH_BL(2)
+CNS_DECLARE_LAUNCH_BL(3)
+CNS_DECLARE_LAUNCH_BL(4)
+#undef CNS_DECLARE_LAUNCH_BL
+
+inline hipError_t launch_rhs_bl(int s, int sv, bool nonuni, int nblocks, hipStream_t st, const Params &P,
+                                const StagePtrs<double> &A) {
+    switch (s) {
+    case 1: return launch_rhs_bl_s1(sv, nonuni, nblocks, st, P, A);
+    case 2: return launch_rhs_bl_s2(sv, nonuni, nblocks, st, P, A);
+    case 3: return launch_rhs_bl_s3(sv, nonuni, nblocks, st, P, A);
+    case 4: return launch_rhs_bl_s4(sv, nonuni, nblocks, st, P, A);
+    }
+    return hipErrorInvalidValue;
+}
+
+inline hipError_t launch_dil_bl(int sv, bool nonuni, int nblocks, hipStream_t st, const Params &P,
+                                const StagePtrs<double> &A, double *dil) {
+    switch (sv) {
+    case 1: return launch_dil_bl_s1(nonuni, nblocks, st, P, A, dil);
+    case 2: return launch_dil_bl_s2(nonuni, nblocks, st, P, A, dil);
+    case 3: return launch_dil_bl_s3(nonuni, nblocks, st, P, A, dil);
+    case 4: return launch_dil_bl_s4(nonuni, nblocks, st, P, A, dil);
+    }
+    return hipErrorInvalidValue;
+}
+
 template <typename R>
 inline hipError_t launch_rhs(int s, int sv, bool wallx, bool nonuni, int nblocks, hipStream_t st, const Params &P,
                              const StagePtrs<R> &A) {

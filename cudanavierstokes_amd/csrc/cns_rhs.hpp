@@ -213,8 +213,10 @@ __device__ __forceinline__ R d1_of(const Params &P, Acc f, int q, double xpi) {
 }
 
 // FLUX = false leaves the convective fluxes out (zero), for a caller that
-// evaluates them separately with dir_flux.
-template <typename R, int S, int SV, int DIR, bool NONUNI, bool UNITV, bool FLUX = true, class Acc>
+// evaluates them separately with dir_flux.  PQ takes the pressure from the
+// staged quantity QP (p itself) instead of forming rho R T at each point.
+template <typename R, int S, int SV, int DIR, bool NONUNI, bool UNITV, bool FLUX = true, bool PQ = false,
+          class Acc>
 __device__ __forceinline__ DirA<R> dir_terms_a(const Params &P, Acc f, const double *vs, double xpi) {
     const R d = R(DIR == 0 ? P.d_dx : (DIR == 1 ? P.d_dy : P.d_dz));
     const R dd = R(DIR == 0 ? P.d_d2x : (DIR == 1 ? P.d_d2y : P.d_d2z));
@@ -255,7 +257,10 @@ __device__ __forceinline__ DirA<R> dir_terms_a(const Params &P, Acc f, const dou
     }
     // advective-order pressure gradient, p = rho R T
     R d1p = R(0);
-    if constexpr (Off<R>::on) {
+    if constexpr (PQ) {
+#pragma unroll
+        for (int k = S; k >= 1; k--) d1p += R(FD<S>::a(k)) * (f(QP, k) - f(QP, -k));
+    } else if constexpr (Off<R>::on) {
         // p' / R = rho' T + rho0 T' on offset storage
         const R T0 = R(P.T0), r0 = R(P.rho0);
 #pragma unroll
@@ -268,7 +273,7 @@ __device__ __forceinline__ DirA<R> dir_terms_a(const Params &P, Acc f, const dou
 #pragma unroll
         for (int k = S; k >= 1; k--) d1p += R(FD<S>::a(k)) * (f(QR, k) * f(QT, k) - f(QR, -k) * f(QT, -k));
     }
-    d1p = d1p * (Rg * xs);
+    d1p = PQ ? d1p * xs : d1p * (Rg * xs);
     const R d1l = d1m * R(P.invPrEc);
 
     R fl[5] = {R(0), R(0), R(0), R(0), R(0)};
@@ -342,10 +347,10 @@ __device__ __forceinline__ void dir_terms_b(const Params &P, const DirA<R> &a, c
 
 // Everything else of one direction's part of the RHS; returns the five
 // partial right-hand sides in the reference's term order.
-template <typename R, int S, int SV, int DIR, bool NONUNI, bool UNITV, class Acc>
+template <typename R, int S, int SV, int DIR, bool NONUNI, bool UNITV, bool PQ = false, class Acc>
 __device__ __forceinline__ void dir_terms(const Params &P, Acc f, const double *vs, double xpi, const R g[9],
                                          const R d2v[3], R out[5]) {
-    dir_terms_b<R, DIR>(P, dir_terms_a<R, S, SV, DIR, NONUNI, UNITV>(P, f, vs, xpi), g, d2v, out);
+    dir_terms_b<R, DIR>(P, dir_terms_a<R, S, SV, DIR, NONUNI, UNITV, true, PQ>(P, f, vs, xpi), g, d2v, out);
 }
 
 __device__ __forceinline__ void block_max_atomic(double v, unsigned long long *out, double *red) {
@@ -741,7 +746,16 @@ __device__ __forceinline__ void rhs_tile(const Params &P, const StagePtrs<R> &A,
 // roles run the same barriers; the role is uniform within a wave.  The sums
 // follow rhs_tile: (x part + y part) + z part, as src/cuda_rhs.cu
 // accumulates them.
-template <typename R, int S, int SV, int TX, int TY, bool WALLX, bool NONUNI, bool UNITV, bool K3>
+// BL: boundary-layer ghosts (src/boundary_condition_x.h, _z.h with
+// boundaryLayer).  The bottom x ghosts are staged by their halo lanes
+// (adiabatic reflection, blowing); the top x ghosts extrapolate the tile's
+// last nodes, so their lanes form them from LDS after the plane is staged,
+// behind a second barrier (the host requires mx to be a multiple of TX, so
+// the last tile holds those nodes).  The z window extrapolates at both
+// ends and carries p as a quantity of its own (QP).  The host also keeps
+// every z chunk at least S planes long, so no window start reaches past an
+// end.
+template <typename R, int S, int SV, int TX, int TY, bool WALLX, bool NONUNI, bool UNITV, bool K3, bool BL = false>
 __device__ __forceinline__ void rhs_tile_split(const Params &P, const StagePtrs<R> &A, const int bb) {
     constexpr int H = S, LX = TX + 2 * H, LY = TY + 2 * H, NT = TX * TY, W = 2 * H + 1;
     constexpr int NL = UNITV ? NQ - 1 : NQ;  // staged quantities (mu rebuilt from T when mu = T/Re)
@@ -752,6 +766,7 @@ __device__ __forceinline__ void rhs_tile_split(const Params &P, const StagePtrs<
     // the roles; with mu = T/Re the plane role evaluates all of it
     constexpr bool YS = !UNITV;
     static_assert(NT % 64 == 0, "each role is made of whole waves");
+    static_assert(!BL || (WALLX && sizeof(R) == 8), "boundary layer: FP64, non-periodic x");
     constexpr unsigned ESZ = sizeof(R);
     __shared__ R sq[2][NL][LY * LX];
     // column -> plane role: du/dz, dv/dz of plane k (used after the barrier
@@ -831,14 +846,16 @@ __device__ __forceinline__ void rhs_tile_split(const Params &P, const StagePtrs<
     };
 
     if (colrole) {
-        R win[NL][W];
+        constexpr int NW = BL ? NL + 1 : NL;  // window quantities (BL: p last)
+        R win[NW][W];
 #pragma unroll
-        for (int q = 0; q < NL; q++)
+        for (int q = 0; q < NW; q++)
 #pragma unroll
             for (int a = 0; a < W; a++) win[q][a] = R(0);
         auto push = [&](const R *t, int a) {
 #pragma unroll
             for (int q = 0; q < NL; q++) win[q][a] = t[q];
+            if constexpr (BL) win[NL][a] = t[QR] * R(P.Rgas) * t[QT];
         };
         // unconditional loads (see rhs_tile)
 #pragma unroll
@@ -852,15 +869,34 @@ __device__ __forceinline__ void rhs_tile_split(const Params &P, const StagePtrs<
         Raw<R> zr;  // prefetched window feed (plane k + H)
         load_raw(A, (long)wrapi(z0 + H, P.mz) * mxy, coff, zr);
         auto FZ = [&](int q, int a) -> R {
-            return (UNITV && q == QM) ? (Off<R>::on ? T0 + win[QT][H + a] : win[QT][H + a]) * invRe : win[q][H + a];
+            if (BL && q == QP) return win[NW - 1][H + a];
+            return (UNITV && q == QM) ? (Off<R>::on ? T0 + win[QT][H + a] : win[QT][H + a]) * invRe
+                                      : win[q < NW ? q : 0][H + a];
         };
         for (int k = z0; k < z1; ++k) {
             const int b = (k - z0) & 1;
             R d1z[3], d2z[3];
             {
-                R t[NQ];
-                to_prim<R, WALLX, UNITV>(P, zr, cside, t);
-                push(t, W - 1);
+                if (BL && k + H >= P.mz) {
+                    // top z ghost: f(mz-1+d) = 2 f(mz-1) - f(mz-1-d) (topBCzExt)
+                    const int d = k + H - (P.mz - 1);
+#pragma unroll
+                    for (int dd = 1; dd <= H; dd++)
+                        if (d == dd)
+#pragma unroll
+                            for (int q = 0; q < NW; q++) win[q][W - 1] = R(2) * win[q][W - 1 - dd] - win[q][W - 1 - 2 * dd];
+                } else {
+                    R t[NQ];
+                    to_prim<R, WALLX, UNITV>(P, zr, cside, t);
+                    push(t, W - 1);
+                }
+                if (BL && k == 0) {
+                    // bottom z ghosts: f(-d) = 2 f(0) - f(d) (botBCzExt)
+#pragma unroll
+                    for (int q = 0; q < NW; q++)
+#pragma unroll
+                        for (int a = 0; a < H; a++) win[q][a] = R(2) * win[q][H] - win[q][2 * H - a];
+                }
 #pragma unroll
                 for (int q = 0; q < NL; q++) sq[b][q][cc] = win[q][H];
                 load_raw(A, (long)wrapi(k + 1 + H, P.mz) * mxy, coff, zr);
@@ -869,6 +905,7 @@ __device__ __forceinline__ void rhs_tile_split(const Params &P, const StagePtrs<
             xgz[b][0][tid] = d1z[0];
             xgz[b][1][tid] = d1z[1];
             __syncthreads();  // plane k staged; du/dz, dv/dz of plane k
+            if constexpr (BL) __syncthreads();  // x-top ghosts of plane k
             if (act) {
                 const auto FX = FXb(b);
                 const auto FY = FYb(b);
@@ -879,7 +916,7 @@ __device__ __forceinline__ void rhs_tile_split(const Params &P, const StagePtrs<
                     const R g[9] = {R(0),   d1_of<R, SV, 0, NONUNI>(P, FX, QV, xpi),
                                     d1_of<R, SV, 0, NONUNI>(P, FX, QW, xpi),
                                     d1y[0], d1y[1], d1y[2], d1z[0], d1z[1], d1z[2]};
-                    dir_terms<R, S, SV, 2, false, UNITV>(P, FZ, nullptr, 1.0, g, d2z, part);
+                    dir_terms<R, S, SV, 2, false, UNITV, BL>(P, FZ, nullptr, 1.0, g, d2z, part);
 #pragma unroll
                     for (int q = 0; q < 5; q++) xyz[b][4 + q][tid] = part[q];
                     dir_terms_b<R, 1>(P, dir_terms_a<R, S, SV, 1, false, UNITV, false>(P, FY, nullptr, 1.0), g, d2y,
@@ -889,13 +926,13 @@ __device__ __forceinline__ void rhs_tile_split(const Params &P, const StagePtrs<
                 } else {
                     const R g[9] = {R(0), R(0), d1_of<R, SV, 0, NONUNI>(P, FX, QW, xpi), R(0), R(0),
                                     d1_of<R, SV, 1, false>(P, FY, QW, xpi), d1z[0], d1z[1], d1z[2]};
-                    dir_terms<R, S, SV, 2, false, UNITV>(P, FZ, nullptr, 1.0, g, d2z, part);
+                    dir_terms<R, S, SV, 2, false, UNITV, BL>(P, FZ, nullptr, 1.0, g, d2z, part);
 #pragma unroll
                     for (int q = 0; q < 5; q++) xyz[b][4 + q][tid] = part[q];
                 }
             }
 #pragma unroll
-            for (int q = 0; q < NL; q++)
+            for (int q = 0; q < NW; q++)
 #pragma unroll
                 for (int a = 0; a < W - 1; a++) win[q][a] = win[q][a + 1];
         }
@@ -917,6 +954,17 @@ __device__ __forceinline__ void rhs_tile_split(const Params &P, const StagePtrs<
         int hside;
         const unsigned hoff = src_offset<WALLX>(P, x0 + hlx - H, y0 + hly - H, ESZ, hside);
         const int hslot = hly * LX + hlx;
+        // BL blowing: sin(omega1 t), sin(10 t) and cos(y/lambda) of this
+        // lane's halo row (y = j / d_dy as src/perturbation.h:50 forms it)
+        double bs1 = 0.0, bs2 = 0.0, bcos = 0.0;
+        if constexpr (BL) {
+            if (P.perturbed) {
+                const double tn = *P.time;
+                bs1 = sin(P.blow_om1 * tn);
+                bs2 = sin(10.0 * tn);
+                bcos = cos(((double)wrapi(y0 + hly - H, P.my) / P.d_dy) / P.blow_lam);
+            }
+        }
         double inv_dx2 = 1.0;
         double vs[2 * SV + 1];
 #pragma unroll
@@ -963,7 +1011,18 @@ __device__ __forceinline__ void rhs_tile_split(const Params &P, const StagePtrs<
             const int b = (k - z0) & 1;
             {
                 R t[NQ];
-                to_prim<R, WALLX, UNITV>(P, hr, hside, t);
+                if constexpr (BL) {
+                    // the source point's primitives; at the bottom the
+                    // adiabatic reflection (the top is formed below)
+                    to_prim<R, false, UNITV>(P, hr, 0, t);
+                    if (hside < 0) {
+                        double ub = 0.0;
+                        const bool blow = P.perturbed && bl_blow(k, bs1, bs2, bcos, ub);
+                        bl_bottom_ghost<R>(P, blow, R(ub), t[QR], t[QU], t[QV], t[QW], t[QT], t[QH]);
+                    }
+                } else {
+                    to_prim<R, WALLX, UNITV>(P, hr, hside, t);
+                }
                 if (hact) {
 #pragma unroll
                     for (int q = 0; q < NL; q++) sq[b][q][hslot] = t[q];
@@ -971,6 +1030,28 @@ __device__ __forceinline__ void rhs_tile_split(const Params &P, const StagePtrs<
                 load_raw(A, (long)wrapi(k + 1, P.mz) * mxy, hoff, hr);
             }
             __syncthreads();  // plane k staged; du/dz, dv/dz of plane k; plane k-1's y stresses and z part
+            if constexpr (BL) {
+                if (hact && hside > 0) {
+                    // top x ghost (topBCxExt on u, v, w, p, T, dil; rho, h,
+                    // mu from the ghost p and T: src/boundary_condition_x.h:60-74,
+                    // src/boundary.h:121-148)
+                    const int d = (x0 + hlx - H) - (P.mx - 1);
+                    const int ls = hly * LX + (P.mx - 1 - x0 + H), is = ls - d;
+                    auto ext = [&](int q) { return R(2) * sq[b][q][ls] - sq[b][q][is]; };
+                    const R Rg = R(P.Rgas);
+                    const R ug = ext(QU), vg = ext(QV), wg = ext(QW), tg = ext(QT), dg = ext(QD);
+                    const R pg = R(2) * (sq[b][QR][ls] * Rg * sq[b][QT][ls]) - sq[b][QR][is] * Rg * sq[b][QT][is];
+                    sq[b][QU][hslot] = ug;
+                    sq[b][QV][hslot] = vg;
+                    sq[b][QW][hslot] = wg;
+                    sq[b][QT][hslot] = tg;
+                    sq[b][QD][hslot] = dg;
+                    sq[b][QH][hslot] = tg * R(P.cpR) + R(0.5) * (ug * ug + vg * vg + wg * wg);
+                    sq[b][QR][hslot] = pg / (Rg * tg);
+                    if constexpr (!UNITV) sq[b][QM % NL][hslot] = R(pow((double)tg, P.viscexp)) * R(P.invRe);
+                }
+                __syncthreads();  // x-top ghosts of plane k
+            }
             if (act) {
                 if (k > z0) finish(k - 1, b ^ 1);
                 const auto FX = FXb(b);
@@ -1021,21 +1102,27 @@ constexpr int rhs_lanes() {
     return (rhs_split<R, UNITV>() ? 2 : 1) * TX * TY;
 }
 
-template <typename R, int S, int SV, int TX, int TY, bool WALLX, bool NONUNI, bool UNITV, bool K3>
+template <typename R, int S, int SV, int TX, int TY, bool WALLX, bool NONUNI, bool UNITV, bool K3, bool BL = false>
 __global__ __launch_bounds__((rhs_lanes<R, UNITV, TX, TY>()), (rhs_waves<R, UNITV>())) void k_rhs(Params P,
                                                                                                    StagePtrs<R> A) {
     const int bb = remap_block(blockIdx.x, gridDim.x, P.xcd_remap);
-    if constexpr (rhs_split<R, UNITV>())
-        rhs_tile_split<R, S, SV, TX, TY, WALLX, NONUNI, UNITV, K3>(P, A, bb);
-    else
+    if constexpr (rhs_split<R, UNITV>()) {
+        rhs_tile_split<R, S, SV, TX, TY, WALLX, NONUNI, UNITV, K3, BL>(P, A, bb);
+    } else {
+        static_assert(!BL, "the boundary layer runs the role-split (FP64) kernel");
         rhs_tile<R, S, SV, TX, TY, WALLX, NONUNI, UNITV, K3>(P, A, bb);
+    }
 }
 
 // Dilatation of the stage input: du/dx + dv/dy + dw/dz at viscous order
 // (src/calc_stress.cu:20-63, 87-96).  Same tiling and prefetch scheme as
 // k_rhs with halo SV: u and v of plane k (tile + halo) in double-buffered
 // LDS, w of the own column in a register window.
-template <typename R, int SV, int TX, int TY, bool WALLX, bool NONUNI>
+// BL: the boundary layer's velocity ghosts (BCxderVel, BCzderVel): u odd
+// about the bottom face with the blowing inside its band, extrapolated about
+// the last node at the top (formed from LDS behind a second barrier, as in
+// k_rhs); w extrapolated about both end planes in the window.
+template <typename R, int SV, int TX, int TY, bool WALLX, bool NONUNI, bool BL = false>
 __device__ __forceinline__ void dil_tile(const Params &P, const StagePtrs<R> &A, R *dil, const int bb) {
     constexpr int H = SV, LX = TX + 2 * H, LY = TY + 2 * H, NT = TX * TY, W = 2 * H + 1;
     constexpr int NH = 2 * H * TY + 2 * H * TX;
@@ -1075,6 +1162,16 @@ __device__ __forceinline__ void dil_tile(const Params &P, const StagePtrs<R> &A,
     const int hslot = hly * LX + hlx;
     // velocities are antisymmetric across an isothermal no-slip wall
     const R csg = cside ? R(-1) : R(1), hsg = hside ? R(-1) : R(1);
+    double bs1 = 0.0, bs2 = 0.0, bcos = 0.0;  // BL blowing (see rhs_tile_split)
+    if constexpr (BL) {
+        static_assert(WALLX && sizeof(R) == 8, "boundary layer: FP64, non-periodic x");
+        if (P.perturbed) {
+            const double tn = *P.time;
+            bs1 = sin(P.blow_om1 * tn);
+            bs2 = sin(10.0 * tn);
+            bcos = cos(((double)wrapi(y0 + hly - H, P.my) / P.d_dy) / P.blow_lam);
+        }
+    }
     double xpi = 1.0;
     if (NONUNI && act) xpi = P.xp[i];
     const R r0 = Off<R>::on ? R(P.rho0) : R(0);  // rho = rho0 + stored value on offset storage
@@ -1111,7 +1208,18 @@ __device__ __forceinline__ void dil_tile(const Params &P, const StagePtrs<R> &A,
         const int b = (k - z0) & 1;
         const long poff = (long)k * mxy;
         const long pnext = (long)wrapi(k + 1, P.mz) * mxy;
-        ww[W - 1] = zr[1] * recip(rho(zr[0]));
+        if (BL && k + H >= P.mz) {
+            const int d = k + H - (P.mz - 1);  // top z ghost (topBCzExt)
+#pragma unroll
+            for (int dd = 1; dd <= H; dd++)
+                if (d == dd) ww[W - 1] = R(2) * ww[W - 1 - dd] - ww[W - 1 - 2 * dd];
+        } else {
+            ww[W - 1] = zr[1] * recip(rho(zr[0]));
+        }
+        if (BL && k == 0) {
+#pragma unroll
+            for (int a = 0; a < H; a++) ww[a] = R(2) * ww[H] - ww[2 * H - a];  // bottom z ghosts (botBCzExt)
+        }
         ldw((long)wrapi(k + 1 + H, P.mz) * mxy, zr);
         {
             const R inv = recip(rho(cr[0]));
@@ -1122,12 +1230,25 @@ __device__ __forceinline__ void dil_tile(const Params &P, const StagePtrs<R> &A,
         {
             const R inv = recip(rho(hr[0]));
             if (hact) {
-                su[b][hslot] = hsg * (hr[1] * inv);
+                R uh = hsg * (hr[1] * inv);
+                if constexpr (BL) {
+                    double ub = 0.0;
+                    if (hside < 0 && P.perturbed && bl_blow(k, bs1, bs2, bcos, ub)) uh = R(ub);
+                }
+                su[b][hslot] = uh;
                 sv[b][hslot] = hsg * (hr[2] * inv);
             }
         }
         ld3(pnext, hoff, hr);
         __syncthreads();
+        if constexpr (BL) {
+            if (hact && hside > 0) {
+                const int d = (x0 + hlx - H) - (P.mx - 1);  // top x ghost of u (topBCxExt)
+                const int ls = hly * LX + (P.mx - 1 - x0 + H);
+                su[b][hslot] = R(2) * su[b][ls] - su[b][ls - d];
+            }
+            __syncthreads();
+        }
         if (act) {
             R a1 = R(0), a2 = R(0), a3 = R(0);
 #pragma unroll
@@ -1152,9 +1273,9 @@ __device__ __forceinline__ void dil_tile(const Params &P, const StagePtrs<R> &A,
     }
 }
 
-template <typename R, int SV, int TX, int TY, bool WALLX, bool NONUNI>
+template <typename R, int SV, int TX, int TY, bool WALLX, bool NONUNI, bool BL = false>
 __global__ __launch_bounds__(TX *TY) void k_dil(Params P, StagePtrs<R> A, R *dil) {
-    dil_tile<R, SV, TX, TY, WALLX, NONUNI>(P, A, dil, remap_block(blockIdx.x, gridDim.x, P.xcd_remap));
+    dil_tile<R, SV, TX, TY, WALLX, NONUNI, BL>(P, A, dil, remap_block(blockIdx.x, gridDim.x, P.xcd_remap));
 }
 
 }  // namespace cns

@@ -58,6 +58,8 @@ class Config:
     twall_bot: float = 1.0
     twall_top: float = 1.0
     dpdz0: float = 0.00372
+    boundary_layer: int = 0  # boundaryLayer (FP64, lone solvers, mx a multiple of 16)
+    perturbed: int = 0  # perturbed: wall-normal blowing at the wall (with boundary_layer)
 
     def to_c(self):
         c = CnsConfig()
@@ -87,6 +89,22 @@ def channel(mx=160, my=128, mz=192, order=6, scheme="rk3", **kw):
     d = dict(mx=mx, my=my, mz=mz, stencil_size=s, stencil_visc=s, periodic_x=0, nonuniform_x=1,
              forcing=1, scheme=SCHEMES.get(scheme, scheme), Lx=2.0, Ly=4 * np.pi / 3, Lz=4 * np.pi,
              Re=3000.0, Pr=0.72, Ma=1.5, viscexp=0.7, stretch=1.8)
+    d.update(kw)
+    return Config(**d)
+
+
+def boundary_layer(mx=240, my=64, mz=2048, order=6, visc_order=4, scheme="rk3", **kw):
+    """Spatially developing boundary layer, the reference's default case
+    (src/globals.h): x wall-normal (adiabatic wall at x = 0, clustered with
+    the half-tanh grid, stretch 5), y spanwise, z streamwise; order 6
+    advective / 4 viscous, Lx 20, Ly 7, Lz 500, Re 1500, Pr 0.75, Ma 0.35,
+    viscexp 1.5, CFL 0.75, the sponge and the wall-normal blowing on.  The
+    sponge strengths and its reference state are set separately
+    (inflow.calculate_sponge)."""
+    d = dict(mx=mx, my=my, mz=mz, stencil_size=order // 2, stencil_visc=visc_order // 2, periodic_x=0,
+             nonuniform_x=1, boundary_layer=1, perturbed=1, sponge=1, forcing=0,
+             scheme=SCHEMES.get(scheme, scheme), check_cfl=100, check_bulk=100, Lx=20.0, Ly=7.0, Lz=500.0,
+             Re=1500.0, Pr=0.75, Ma=0.35, viscexp=1.5, cfl=0.75, stretch=5.0)
     d.update(kw)
     return Config(**d)
 

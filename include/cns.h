@@ -69,6 +69,12 @@ typedef struct cns_config {
     double stretch;       /* tanh stretching of x (stretch) */
     double twall_bot, twall_top; /* wall temperatures (TwallBot, TwallTop) */
     double dpdz0;         /* initial forcing (src/cuda_utils.cu:68-70) */
+    int boundary_layer;   /* boundaryLayer: with periodic_x 0, an adiabatic no-slip wall at x = 0, linear
+                             extrapolation at the x top and both z ends, the wall-clustered half-tanh grid
+                             (src/boundary_condition_{x,z}.h, src/init.cpp:41-50); FP64 storage, lone
+                             solvers (not in batches), nx a multiple of 16, nz > 2*stencil_size */
+    int perturbed;        /* perturbed: wall-normal blowing at x = 0 on z planes 90..130
+                             (src/perturbation.h:25-53) */
 } cns_config;
 
 typedef struct cns_solver cns_solver;

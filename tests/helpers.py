@@ -8,7 +8,7 @@ from tests import cases
 _MAP = {"stencil_size": "s", "stencil_visc": "sv"}
 _COMMON = ["mx", "my", "mz", "periodic_x", "nonuniform_x", "forcing", "sponge", "scheme", "check_cfl",
            "check_bulk", "fix_y_work", "Lx", "Ly", "Lz", "Re", "Pr", "Ma", "gam", "viscexp", "cfl", "stretch",
-           "twall_bot", "twall_top", "dpdz0"]
+           "twall_bot", "twall_top", "dpdz0", "boundary_layer", "perturbed"]
 
 
 def oracle_from(config):

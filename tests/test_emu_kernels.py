@@ -11,7 +11,7 @@ EMU = os.path.join(ROOT, "tests", "emu")
 
 
 def test_kernel_sources_on_cpu_emulation():
-    subprocess.run(["make", "-s", "-C", EMU, "_build/libcns_emu.so"], check=True)
+    subprocess.run(["make", "-s", "-j8", "-C", EMU, "_build/libcns_emu.so"], check=True)
     lib = os.path.join(EMU, "_build", "libcns_emu.so")
     env = dict(os.environ, OMP_NUM_THREADS="2")
     r = subprocess.run([sys.executable, os.path.join(EMU, "run_emu_parity.py"), lib], capture_output=True,
