@@ -29,7 +29,7 @@ class CnsConfig(C.Structure):
         ("reynolds", C.c_double), ("prandtl", C.c_double), ("mach", C.c_double), ("gamma", C.c_double),
         ("visc_exp", C.c_double), ("cfl", C.c_double), ("stretch", C.c_double),
         ("twall_bot", C.c_double), ("twall_top", C.c_double), ("dpdz0", C.c_double),
-        ("boundary_layer", C.c_int), ("perturbed", C.c_int),
+        ("boundary_layer", C.c_int), ("wall_blowing", C.c_int),
     ]
 
 
@@ -44,7 +44,7 @@ class CnsControl(C.Structure):
 # Python Config names (the reference's macro names) -> C struct fields
 C_NAMES = {"mx": "nx", "my": "ny", "mz": "nz", "forcing": "forcing_z", "Lx": "len_x", "Ly": "len_y",
            "Lz": "len_z", "Re": "reynolds", "Pr": "prandtl", "Ma": "mach", "gam": "gamma",
-           "viscexp": "visc_exp"}
+           "viscexp": "visc_exp", "perturbed": "wall_blowing"}
 
 
 EXPORTS = [

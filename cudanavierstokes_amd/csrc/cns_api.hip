@@ -360,7 +360,7 @@ static void fill_params(cns_solver *s) {
     P.twall_bot = c.twall_bot;
     P.twall_top = c.twall_top;
     P.blayer = c.boundary_layer ? 1 : 0;
-    P.perturbed = (c.boundary_layer && c.perturbed) ? 1 : 0;
+    P.perturbed = (c.boundary_layer && c.wall_blowing) ? 1 : 0;
     P.blow_om1 = c.reynolds * 121.e-6;
     P.blow_lam = c.len_y / (2.0 * M_PI);
     for (int i = 0; i < 4; i++) {
@@ -413,7 +413,7 @@ extern "C" int cns_config_init(cns_config *c) {
     c->twall_top = 1.0;
     c->dpdz0 = 0.00372;
     c->boundary_layer = 0;
-    c->perturbed = 0;
+    c->wall_blowing = 0;
     return CNS_OK;
 }
 

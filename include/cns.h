@@ -73,8 +73,8 @@ typedef struct cns_config {
                              extrapolation at the x top and both z ends, the wall-clustered half-tanh grid
                              (src/boundary_condition_{x,z}.h, src/init.cpp:41-50); FP64 storage, lone
                              solvers (not in batches), nx a multiple of 16, nz > 2*stencil_size */
-    int perturbed;        /* perturbed: wall-normal blowing at x = 0 on z planes 90..130
-                             (src/perturbation.h:25-53) */
+    int wall_blowing;     /* perturbed: wall-normal blowing at x = 0 on z planes 90..130
+                             (src/perturbation.h:25-53); named apart from the reference's macro */
 } cns_config;
 
 typedef struct cns_solver cns_solver;

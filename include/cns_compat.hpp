@@ -89,6 +89,10 @@ inline void setGPUParameters(Communicator) {
     c.nonuniform_x = nonUniformX ? 1 : 0;
     c.forcing_z = forcing ? 1 : 0;
     c.sponge = boundaryLayer ? 1 : 0;
+    c.boundary_layer = boundaryLayer ? 1 : 0;
+#ifdef perturbed
+    c.wall_blowing = perturbed ? 1 : 0;
+#endif
     c.scheme = lowStorage ? CNS_RK3_LOWSTORAGE : CNS_RK3_KUTTA;
     c.check_cfl = checkCFLcondition;
     c.check_bulk = checkBulk;

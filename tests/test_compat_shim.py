@@ -72,6 +72,7 @@ def test_compat_shim_compiles_after_reference_macros(tmp_path):
 
 
 BL_CASE = CASE.replace("#define boundaryLayer (false)", "#define boundaryLayer (true)").replace(
+    "#define perturbed (false)", "#define perturbed (true)").replace(
     "#define forcing (true)", "#define forcing (false)").replace(
     "double r[mx*my*mz]", "double x[mx], z[mz_tot];\ndouble r[mx*my*mz]").replace(
     "#include \"cns_compat.hpp\"",
