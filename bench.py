@@ -159,16 +159,22 @@ def other_configs(cns, steps):
     """Throughput of the other reference configurations (steps with the
     state resident, launch shape tuned): the 64^3 4th-order Taylor-Green case
     with the low-storage RK3, and the 160x128x192 isothermal-wall channel at
-    6th order (x wall-normal, stretched; forcing on), FP64 and FP32 storage."""
+    6th order (x wall-normal, stretched; forcing on), FP64 and FP32 storage,
+    and the reference's default boundary layer (240x64x2048, order 6/4,
+    sponge and wall-normal blowing, FP64) from its Blasius inflow state."""
+    from cudanavierstokes_amd import inflow
     out = []
     cases = [("tgv64_o4_rk3_fp64", cns.taylor_green(n=64, order=4, scheme="rk3"), "tgv"),
              ("channel160x128x192_o6_rk3_fp64", cns.channel(mx=160, my=128, mz=192, order=6, scheme="rk3"), "channel"),
              ("channel160x128x192_o6_rk3_fp32",
-              cns.channel(mx=160, my=128, mz=192, order=6, scheme="rk3", precision=1), "channel")]
+              cns.channel(mx=160, my=128, mz=192, order=6, scheme="rk3", precision=1), "channel"),
+             ("boundary_layer240x64x2048_o6v4_rk3_fp64", cns.boundary_layer(), "blayer")]
     for name, cfg, kind in cases:
         s = cns.Solver(cfg)
         if kind == "tgv":
             s.init_taylor_green()
+        elif kind == "blayer":
+            inflow.calculate_sponge(s, set_state=True)
         else:
             s.init_channel(seed=1)
         pencil = s.tune()

@@ -162,6 +162,9 @@ struct cns_solver {
     Params base{};
     // profiling
     bool sync_check = false;  // CNS_SYNC_CHECK=1: synchronise and check after every launch
+    // CNS_DIL_TILES=wide: the dilatation kernel always on its own 32 x 16
+    // tile grid; =rhs: always on the RHS kernel's tiles; unset: by size
+    int dil_tiles = 0;
     bool prof = false;
     std::vector<hipEvent_t> ev;
     int prof_n = 0;
@@ -539,6 +542,8 @@ extern "C" int cns_create(const cns_config *cfg, cns_solver **out) {
     {
         const char *sc = std::getenv("CNS_SYNC_CHECK");
         s->sync_check = sc && sc[0] == '1';
+        const char *dt = std::getenv("CNS_DIL_TILES");
+        s->dil_tiles = dt ? (std::strcmp(dt, "wide") == 0 ? 1 : (std::strcmp(dt, "rhs") == 0 ? 2 : 0)) : 0;
     }
     s->S = cfg->stencil_size;
     s->SV = cfg->stencil_visc;
@@ -946,7 +951,7 @@ static int run_stage_on(cns_solver *s, const R *qin, R *dil, Params P, StagePtrs
     // workgroups on 32 x 16 tiles (fewer halo points per point), with its own
     // tile grid and z chunks: about 1024 workgroups, chunks of 8..the RHS's
     // (small grids, where that leaves fewer than 512 workgroups, keep the
-    // RHS kernel's tiles)
+    // RHS kernel's tiles; CNS_DIL_TILES overrides the choice)
     Params Pd = P;
     Pd.tile_x = 32;
     Pd.ntx = (s->cfg.nx + 31) / 32;
@@ -957,7 +962,7 @@ static int run_stage_on(cns_solver *s, const R *qin, R *dil, Params P, StagePtrs
         Pd.nzc = (s->cfg.nz + Pd.zc - 1) / Pd.zc;
     }
     int dil_blocks = Pd.ntx * Pd.nty * Pd.nzc;
-    if (dil_blocks < 512) {
+    if (s->dil_tiles == 2 || (dil_blocks < 512 && s->dil_tiles != 1)) {
         Pd = P;
         dil_blocks = s->nblocks;
     }

@@ -71,6 +71,62 @@ def test_rhs_parity(name, kind, kw):
     sol.close()
 
 
+# The dilatation kernel runs on its own 32 x 16 tile grid only above a size
+# threshold (512 workgroups); CNS_DIL_TILES=wide forces that grid on the
+# small cases too: walls, stretched x (the metric in FP64 on FP32 storage),
+# partial tiles, FP32 offset storage.  Each case is compared with the oracle
+# and with the same solver on the RHS kernel's tiles (the dilatation's
+# arithmetic per point is the same in both: FP64 to 1e-13, FP32 to 1e-6).
+WIDE_CASES = ["random_box_s4_fix", "channel_s3", "channel_s4_uniform_wall", "periodic_stretched"]
+_BY_NAME = {c[0]: c for c in RHS_CASES}
+
+
+def _rhs_with_tiles(cfg, kind, mode, monkeypatch):
+    monkeypatch.setenv("CNS_DIL_TILES", mode)
+    orc, sol, _ = _pair(cfg, kind)
+    try:
+        return sol.rhs(), orc
+    finally:
+        sol.close()
+        monkeypatch.delenv("CNS_DIL_TILES")
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("name", WIDE_CASES)
+def test_rhs_parity_wide_dilatation_tiles(name, precision, monkeypatch):
+    _, kind, kw = _BY_NAME[name]
+    cfg = cns.Config(**kw).replace(precision=precision)
+    rw, orc = _rhs_with_tiles(cfg, kind, "wide", monkeypatch)
+    rr, _ = _rhs_with_tiles(cfg, kind, "rhs", monkeypatch)
+    ro = orc.rhs()
+    e_or = cases.rel_errors(rw, ro)
+    e_tiles = cases.rel_errors(rw, rr)
+    flush_print(name, precision, "vs oracle", e_or, "vs RHS tiles", e_tiles)
+    if precision == 0:
+        assert_close(e_or, RHS_L2, RHS_LINF, f"wide dilatation {name}")
+        assert_close(e_tiles, 1e-13, 1e-12, f"wide vs RHS tiles {name}")
+    else:
+        # one FP32 RHS of these Mach 0.3-1.5 cases against the FP64 oracle
+        assert_close(e_or, 1e-5, 1e-4, f"wide dilatation fp32 {name}")
+        assert_close(e_tiles, 1e-6, 1e-5, f"wide vs RHS tiles fp32 {name}")
+
+
+def test_rhs_128_error_is_the_operators_rounding(monkeypatch):
+    """The 128^3 case's larger relative error (the 1e-11 bound above) is
+    rounding of the same operators, not the wide dilatation grid: the RHS
+    kernel's own tiles give the same error, and the two paths agree."""
+    _, kind, kw = _BY_NAME["tgv128_s2_wide_dilatation"]
+    cfg = cns.Config(**kw)
+    rw, orc = _rhs_with_tiles(cfg, kind, "wide", monkeypatch)
+    rr, _ = _rhs_with_tiles(cfg, kind, "rhs", monkeypatch)
+    ro = orc.rhs()
+    ew, er, et = cases.rel_errors(rw, ro), cases.rel_errors(rr, ro), cases.rel_errors(rw, rr)
+    flush_print("128^3 wide", ew, "rhs tiles", er, "between", et)
+    for q in ew:
+        assert ew[q][0] <= 1.5 * er[q][0] + 1e-14, (q, ew[q], er[q])
+    assert_close(et, 1e-13, 1e-12, "128^3 wide vs RHS tiles")
+
+
 def test_grid_matches_oracle():
     for kw in (dict(mx=24, my=16, mz=20, nonuniform_x=1, periodic_x=0, Lx=2.0, stretch=1.8, stencil_size=3,
                     stencil_visc=3),
