@@ -155,6 +155,9 @@ __device__ __forceinline__ int wrapi(int a, int n) {
     a %= n;
     return a < 0 ? a + n : a;
 }
+// wrapi for a in [-n, 2n), without the division: z-plane indices within a
+// stencil half width of the domain (n > 2S holds for every grid)
+__device__ __forceinline__ int wrap1(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
 
 // Consecutive workgroup ids are spread round-robin over the 8 XCDs; remap so
 // that each XCD receives a contiguous range of tiles (neighbouring tiles share

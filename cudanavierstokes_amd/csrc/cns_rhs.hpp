@@ -632,13 +632,13 @@ __device__ __forceinline__ void rhs_tile(const Params &P, const StagePtrs<R> &A,
 #pragma unroll
     for (int a = 0; a < W - 1; a++) {
         Raw<R> x;
-        load_raw(A, (long)wrapi(z0 - H + a, P.mz) * mxy, coff, x);
+        load_raw(A, (long)wrap1(z0 - H + a, P.mz) * mxy, coff, x);
         R t[NQ];
         to_prim<R, WALLX, UNITV>(P, x, cside, t);
         push(t, a);
     }
     Raw<R> zr, hr;  // prefetched: window feed (plane k + H) and halo point (plane k)
-    load_raw(A, (long)wrapi(z0 + H, P.mz) * mxy, coff, zr);
+    load_raw(A, (long)wrap1(z0 + H, P.mz) * mxy, coff, zr);
     load_raw(A, (long)z0 * mxy, hoff, hr);
     double vmax = 0.0;
     const double dtd = (P.write_q || P.write_acc) ? *P.dt : 0.0;
@@ -663,7 +663,7 @@ __device__ __forceinline__ void rhs_tile(const Params &P, const StagePtrs<R> &A,
             push(t, W - 1);
 #pragma unroll
             for (int q = 0; q < NL; q++) sq[b][q][cc] = win[q][H];
-            load_raw(A, (long)wrapi(k + 1 + H, P.mz) * mxy, coff, zr);
+            load_raw(A, (long)wrap1(k + 1 + H, P.mz) * mxy, coff, zr);
         }
         __builtin_amdgcn_sched_barrier(0);
         {
@@ -673,7 +673,7 @@ __device__ __forceinline__ void rhs_tile(const Params &P, const StagePtrs<R> &A,
 #pragma unroll
                 for (int q = 0; q < NL; q++) sq[b][q][hslot] = t[q];
             }
-            load_raw(A, (long)wrapi(k + 1, P.mz) * mxy, hoff, hr);
+            load_raw(A, (long)wrap1(k + 1, P.mz) * mxy, hoff, hr);
         }
         __syncthreads();
         if constexpr (!early_ops) load_update_operands();
@@ -861,13 +861,13 @@ __device__ __forceinline__ void rhs_tile_split(const Params &P, const StagePtrs<
 #pragma unroll
         for (int a = 0; a < W - 1; a++) {
             Raw<R> x;
-            load_raw(A, (long)wrapi(z0 - H + a, P.mz) * mxy, coff, x);
+            load_raw(A, (long)wrap1(z0 - H + a, P.mz) * mxy, coff, x);
             R t[NQ];
             to_prim<R, WALLX, UNITV>(P, x, cside, t);
             push(t, a);
         }
         Raw<R> zr;  // prefetched window feed (plane k + H)
-        load_raw(A, (long)wrapi(z0 + H, P.mz) * mxy, coff, zr);
+        load_raw(A, (long)wrap1(z0 + H, P.mz) * mxy, coff, zr);
         auto FZ = [&](int q, int a) -> R {
             if (BL && q == QP) return win[NW - 1][H + a];
             return (UNITV && q == QM) ? (Off<R>::on ? T0 + win[QT][H + a] : win[QT][H + a]) * invRe
@@ -899,7 +899,7 @@ __device__ __forceinline__ void rhs_tile_split(const Params &P, const StagePtrs<
                 }
 #pragma unroll
                 for (int q = 0; q < NL; q++) sq[b][q][cc] = win[q][H];
-                load_raw(A, (long)wrapi(k + 1 + H, P.mz) * mxy, coff, zr);
+                load_raw(A, (long)wrap1(k + 1 + H, P.mz) * mxy, coff, zr);
             }
             vel_stencils<R, SV, 2, false>(P, FZ, nullptr, 1.0, d1z, d2z);
             xgz[b][0][tid] = d1z[0];
@@ -1027,7 +1027,7 @@ __device__ __forceinline__ void rhs_tile_split(const Params &P, const StagePtrs<
 #pragma unroll
                     for (int q = 0; q < NL; q++) sq[b][q][hslot] = t[q];
                 }
-                load_raw(A, (long)wrapi(k + 1, P.mz) * mxy, hoff, hr);
+                load_raw(A, (long)wrap1(k + 1, P.mz) * mxy, hoff, hr);
             }
             __syncthreads();  // plane k staged; du/dz, dv/dz of plane k; plane k-1's y stresses and z part
             if constexpr (BL) {
@@ -1125,8 +1125,11 @@ __global__ __launch_bounds__((rhs_lanes<R, UNITV, TX, TY>()), (rhs_waves<R, UNIT
 template <typename R, int SV, int TX, int TY, bool WALLX, bool NONUNI, bool BL = false>
 __device__ __forceinline__ void dil_tile(const Params &P, const StagePtrs<R> &A, R *dil, const int bb) {
     constexpr int H = SV, LX = TX + 2 * H, LY = TY + 2 * H, NT = TX * TY, W = 2 * H + 1;
-    constexpr int NH = 2 * H * TY + 2 * H * TX;
-    static_assert(NH <= NT, "one halo point per lane");
+    // halo lanes: the x-halo points (which need u only) first, padded to
+    // whole waves, then the y-halo points (v only), so that each wave loads
+    // density and one momentum field, the same one for all its lanes
+    constexpr int NX = 2 * H * TY, XPAD = (NX + 63) / 64 * 64, NY = 2 * H * TX;
+    static_assert(XPAD + NY <= NT, "one halo point per lane");
     constexpr unsigned ESZ = sizeof(R);
     __shared__ R su[2][LY * LX], sv[2][LY * LX];
     const int tile_x = bb % P.ntx;
@@ -1145,18 +1148,21 @@ __device__ __forceinline__ void dil_tile(const Params &P, const StagePtrs<R> &A,
 
     int cside;
     const unsigned coff = src_offset<WALLX>(P, i, j, ESZ, cside);
-    const bool hact = tid < NH;
+    const bool xh = __builtin_amdgcn_readfirstlane(tid) < XPAD;  // uniform per wave
+    const bool hact = xh ? tid < NX : tid - XPAD < NY;
     int hlx, hly;
-    if (tid < 2 * H * TY) {
-        hly = H + tid / (2 * H);
-        const int c2 = tid % (2 * H);
+    if (xh) {
+        const int t2 = hact ? tid : 0;
+        hly = H + t2 / (2 * H);
+        const int c2 = t2 % (2 * H);
         hlx = c2 < H ? c2 : TX + c2;
     } else {
-        const int s2 = hact ? tid - 2 * H * TY : 0;
+        const int s2 = hact ? tid - XPAD : 0;
         hlx = H + s2 % TX;
         const int rr = s2 / TX;
         hly = rr < H ? rr : TY + rr;
     }
+    const int hq = xh ? 1 : 2;  // momentum field of this wave's halo points
     int hside;
     const unsigned hoff = src_offset<WALLX>(P, x0 + hlx - H, y0 + hly - H, ESZ, hside);
     const int hslot = hly * LX + hlx;
@@ -1177,72 +1183,81 @@ __device__ __forceinline__ void dil_tile(const Params &P, const StagePtrs<R> &A,
     const R r0 = Off<R>::on ? R(P.rho0) : R(0);  // rho = rho0 + stored value on offset storage
     auto rho = [&](R stored) -> R { return Off<R>::on ? r0 + stored : stored; };
 
-    // u, v of the own column / halo point at plane poff, w of the own column
-    auto ld3 = [&](long poff, unsigned o, R v3[3]) {
-        const R *b = A.qin + poff;
-        v3[0] = ld_at(b, o);
-        v3[1] = ld_at(b + fs, o);
-        v3[2] = ld_at(b + 2 * fs, o);
-    };
+    // the own column: rho and rho w of the window plane, rho u and rho v of
+    // the next staged plane (its density is the window's); a halo point:
+    // rho and this wave's momentum field
     auto ldw = [&](long poff, R v2[2]) {
         const R *b = A.qin + poff;
         v2[0] = ld_at(b, coff);
         v2[1] = ld_at(b + 3 * fs, coff);
     };
+    auto ldc = [&](long poff, R v2[2]) {
+        const R *b = A.qin + poff;
+        v2[0] = ld_at(b + fs, coff);
+        v2[1] = ld_at(b + 2 * fs, coff);
+    };
+    auto ldh = [&](long poff, R v2[2]) {
+        const R *b = A.qin + poff;
+        v2[0] = ld_at(b, hoff);
+        v2[1] = ld_at(b + hq * fs, hoff);
+    };
     // unconditional loads (see rhs_tile): lanes past the edge and lanes
-    // without a halo point load valid points they do not use
-    R ww[W];
+    // without a halo point load valid points they do not use.  Window: w and
+    // 1/rho of the own column for planes k-H..k+H.
+    R ww[W], ri[W];
 #pragma unroll
     for (int a = 0; a < W - 1; a++) {
         R t[2];
-        ldw((long)wrapi(z0 - H + a, P.mz) * mxy, t);
-        ww[a] = t[1] * recip(rho(t[0]));
+        ldw((long)wrap1(z0 - H + a, P.mz) * mxy, t);
+        ri[a] = recip(rho(t[0]));
+        ww[a] = t[1] * ri[a];
     }
-    ww[W - 1] = R(0);
-    R zr[2], cr[3], hr[3];
-    ldw((long)wrapi(z0 + H, P.mz) * mxy, zr);
-    ld3((long)z0 * mxy, coff, cr);
-    ld3((long)z0 * mxy, hoff, hr);
+    ww[W - 1] = ri[W - 1] = R(0);
+    R zr[2], cr[2], hr[2];
+    ldw((long)wrap1(z0 + H, P.mz) * mxy, zr);
+    ldc((long)z0 * mxy, cr);
+    ldh((long)z0 * mxy, hr);
 
     for (int k = z0; k < z1; ++k) {
         const int b = (k - z0) & 1;
         const long poff = (long)k * mxy;
-        const long pnext = (long)wrapi(k + 1, P.mz) * mxy;
+        const long pnext = (long)wrap1(k + 1, P.mz) * mxy;
         if (BL && k + H >= P.mz) {
             const int d = k + H - (P.mz - 1);  // top z ghost (topBCzExt)
 #pragma unroll
             for (int dd = 1; dd <= H; dd++)
                 if (d == dd) ww[W - 1] = R(2) * ww[W - 1 - dd] - ww[W - 1 - 2 * dd];
         } else {
-            ww[W - 1] = zr[1] * recip(rho(zr[0]));
+            ri[W - 1] = recip(rho(zr[0]));
+            ww[W - 1] = zr[1] * ri[W - 1];
         }
         if (BL && k == 0) {
 #pragma unroll
             for (int a = 0; a < H; a++) ww[a] = R(2) * ww[H] - ww[2 * H - a];  // bottom z ghosts (botBCzExt)
         }
-        ldw((long)wrapi(k + 1 + H, P.mz) * mxy, zr);
+        ldw((long)wrap1(k + 1 + H, P.mz) * mxy, zr);
+        su[b][cc] = csg * (cr[0] * ri[H]);
+        sv[b][cc] = csg * (cr[1] * ri[H]);
+        ldc(pnext, cr);
         {
-            const R inv = recip(rho(cr[0]));
-            su[b][cc] = csg * (cr[1] * inv);
-            sv[b][cc] = csg * (cr[2] * inv);
-        }
-        ld3(pnext, coff, cr);
-        {
-            const R inv = recip(rho(hr[0]));
+            const R vh = hsg * (hr[1] * recip(rho(hr[0])));
             if (hact) {
-                R uh = hsg * (hr[1] * inv);
-                if constexpr (BL) {
-                    double ub = 0.0;
-                    if (hside < 0 && P.perturbed && bl_blow(k, bs1, bs2, bcos, ub)) uh = R(ub);
+                if (xh) {
+                    R uh = vh;
+                    if constexpr (BL) {
+                        double ub = 0.0;
+                        if (hside < 0 && P.perturbed && bl_blow(k, bs1, bs2, bcos, ub)) uh = R(ub);
+                    }
+                    su[b][hslot] = uh;
+                } else {
+                    sv[b][hslot] = vh;
                 }
-                su[b][hslot] = uh;
-                sv[b][hslot] = hsg * (hr[2] * inv);
             }
         }
-        ld3(pnext, hoff, hr);
+        ldh(pnext, hr);
         __syncthreads();
         if constexpr (BL) {
-            if (hact && hside > 0) {
+            if (hact && xh && hside > 0) {
                 const int d = (x0 + hlx - H) - (P.mx - 1);  // top x ghost of u (topBCxExt)
                 const int ls = hly * LX + (P.mx - 1 - x0 + H);
                 su[b][hslot] = R(2) * su[b][ls] - su[b][ls - d];
@@ -1269,7 +1284,10 @@ __device__ __forceinline__ void dil_tile(const Params &P, const StagePtrs<R> &A,
             st_at(dil + poff, coff, a1 + a2 + a3);
         }
 #pragma unroll
-        for (int a = 0; a < W - 1; a++) ww[a] = ww[a + 1];
+        for (int a = 0; a < W - 1; a++) {
+            ww[a] = ww[a + 1];
+            ri[a] = ri[a + 1];
+        }
     }
 }
 
