@@ -312,9 +312,9 @@ static void launch_geometry(cns_solver *s) {
     s->ntx = (c.nx + tx - 1) / tx;
     s->nty = (c.ny + TILE_Y - 1) / TILE_Y;
     const long cols = (long)s->ntx * s->nty;
-    // workgroups to aim at: one round, i.e. one workgroup per CU (512 lanes
-    // at two waves per SIMD), or two for FP32 with a general viscosity law
-    // (256 lanes at two waves per SIMD)
+    // workgroups to aim at: one round, i.e. one 512-lane workgroup per CU at
+    // two waves per SIMD, or two rounds of shorter marches for FP32 with a
+    // general viscosity law (measured faster there)
     const long target = (fp32 && !unitv) ? 512 : 256;
     long zc = ((long)c.nz * cols + target - 1) / target;
     zc = std::max<long>(zc, 8);

@@ -6,17 +6,17 @@
 
 namespace cns {
 
-// Tiles are TX x 16 columns.  The FP32 kernels with mu = T/Re take 32 x 16
-// tiles in 512-lane workgroups (384 halo points for 512 interior ones
-// instead of 256 for 256); the other FP32 kernels 16 x 16 tiles in 256-lane
-// workgroups (32 x 16 measured slower for them); the FP64 role-split kernel
-// 16 x 16 tiles with two lanes per column (512 lanes).
+// Tiles are TX x 16 columns.  The FP32 kernels take 32 x 16 tiles in
+// 512-lane workgroups (384 halo points for 512 interior ones instead of 256
+// for 256; measured 11-14% faster than 16 x 16 on order-6 channels, and the
+// same at order 8); the FP64 role-split kernel 16 x 16 tiles with two lanes
+// per column (512 lanes).
 constexpr int TILE_Y = 16;
 template <typename R, bool UNITV>
 constexpr int tile_x() {
-    return (sizeof(R) == 4 && UNITV) ? 32 : 16;
+    return sizeof(R) == 4 ? 32 : 16;
 }
-inline int tile_x_of(bool fp32, bool unitv) { return (fp32 && unitv) ? 32 : 16; }
+inline int tile_x_of(bool fp32, bool unitv) { return fp32 ? 32 : 16; }
 
 #define CNS_DECLARE_LAUNCH(N)                                                                                  \
     template <typename R>                                                                                      \

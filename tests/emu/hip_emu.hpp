@@ -79,6 +79,7 @@ inline float __builtin_amdgcn_exp2f(float x) { return std::exp2(x); }
 
 // scheduling hints and wave-uniform reads have no effect on the CPU
 #define __builtin_amdgcn_sched_barrier(m) ((void)(m))
+#define __builtin_amdgcn_s_setprio(p) ((void)(p))
 inline int __builtin_amdgcn_readfirstlane(int v) { return v; }
 
 inline double __shfl_xor(double v, int off) {
