@@ -91,7 +91,7 @@ __device__ __forceinline__ void cfl_point(const Params &P, const StagePtrs<R> &A
     cmax = dmax(cmax, dmax(c1, dmax(c2, c3)));
     if (visc_from_state) {
         const double t = P.cvInv * ien;
-        const double mu = (P.unit_visc ? t : pow(t, P.viscexp)) / P.Re;
+        const double mu = (P.unit_visc ? t : visc_pow(t, P.viscexp)) / P.Re;
         const double v1 = mu / (dx * dx), v2 = mu * P.d_d2y, v3 = mu * P.d_d2z;
         vmax = dmax(vmax, dmax(v1, dmax(v2, v3)));
     }

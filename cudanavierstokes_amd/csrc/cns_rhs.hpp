@@ -1052,7 +1052,7 @@ __device__ __forceinline__ void rhs_tile_split(const Params &P, const StagePtrs<
                     sq[b][QD][hslot] = dg;
                     sq[b][QH][hslot] = tg * R(P.cpR) + R(0.5) * (ug * ug + vg * vg + wg * wg);
                     sq[b][QR][hslot] = pg / (Rg * tg);
-                    if constexpr (!UNITV) sq[b][QM % NL][hslot] = R(pow((double)tg, P.viscexp)) * R(P.invRe);
+                    if constexpr (!UNITV) sq[b][QM % NL][hslot] = R(visc_pow((double)tg, P.viscexp)) * R(P.invRe);
                 }
                 __syncthreads();  // x-top ghosts of plane k
             }
