@@ -51,7 +51,7 @@ EXPORTS = [
     "cns_config_init", "cns_create", "cns_destroy", "cns_last_error", "cns_get_grid",
     "cns_fill_taylor_green", "cns_fill_channel", "cns_init_taylor_green", "cns_init_channel", "cns_set_state", "cns_get_state", "cns_rhs", "cns_rhs_noalloc",
     "cns_step", "cns_set_sponge", "cns_get_scalars", "cns_sync", "cns_profile_enable",
-    "cns_profile_read", "cns_time_steps", "cns_device_bytes", "cns_mem_info", "cns_sponge_from_profile",
+    "cns_profile_read", "cns_time_steps", "cns_device_bytes", "cns_mem_info", "cns_release_cache", "cns_sponge_from_profile",
     "cns_write_fields", "cns_read_fields", "cns_write_grid", "cns_get_control", "cns_set_control",
     "cns_channel_profiles", "cns_friction_reynolds", "cns_flow_stats", "cns_set_pencil", "cns_tune",
     "cns_step_stats",
@@ -100,6 +100,7 @@ def load(path=None):
     L.cns_time_steps.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
     L.cns_device_bytes.argtypes = [vp, C.POINTER(C.c_uint64)]
     L.cns_mem_info.argtypes = [C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    L.cns_release_cache.argtypes = [C.c_int]
     L.cns_sponge_from_profile.argtypes = [C.c_int, dp, dp, dp, dp, C.c_int, dp, C.c_int, dp, C.c_double,
                                           C.c_double, C.c_double, C.c_double, C.c_double, vp, dp, dp, dp, dp,
                                           dp, dp, dp]

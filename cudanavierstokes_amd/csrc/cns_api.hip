@@ -25,7 +25,9 @@
 #include <string>
 #include <thread>
 #include <type_traits>
+#include <map>
 #include <mutex>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/cns.h"
@@ -205,14 +207,114 @@ static int fail(cns_solver *s, int code, const std::string &msg) {
     return code;
 }
 
+// Device blocks that a solver or batch releases stay cached per device and
+// byte size, and the next allocation of the same size on that device takes
+// one: hipFree waits for the whole device, and a serving loop closes and
+// creates solvers of a few recurring sizes between its ticks.  At most
+// cache_cap() bytes stay cached per device (CNS_CACHE_BYTES, default 16 GiB),
+// the oldest blocks leave first; cns_release_cache frees them.  Blocks are
+// released only after their work has finished (cns_destroy and
+// cns_batch_destroy synchronise their streams first).
+namespace {
+struct CachedBlock {
+    int dev;
+    size_t bytes;
+    void *p;
+};
+std::mutex g_cache_mu;
+std::vector<CachedBlock> g_cache;                  // oldest first
+std::unordered_map<void *, size_t> g_block_bytes;  // live blocks from dev_alloc
+std::map<int, size_t> g_cached_bytes;              // per device
+}  // namespace
+
+static size_t cache_cap() {
+    static const size_t cap = [] {
+        const char *e = std::getenv("CNS_CACHE_BYTES");
+        return e ? (size_t)std::strtoull(e, nullptr, 10) : ((size_t)16 << 30);
+    }();
+    return cap;
+}
+
+// frees cached blocks of device dev (the current device) until at most keep
+// bytes stay; call with g_cache_mu held
+static void cache_shrink(int dev, size_t keep) {
+    for (size_t i = 0; i < g_cache.size() && g_cached_bytes[dev] > keep;) {
+        if (g_cache[i].dev == dev) {
+            (void)hipFree(g_cache[i].p);
+            g_cached_bytes[dev] -= g_cache[i].bytes;
+            g_cache.erase(g_cache.begin() + (long)i);
+        } else {
+            i++;
+        }
+    }
+}
+
+// allocation on the current device dev, from the cache when a block of the
+// same size is there
+static hipError_t dev_alloc(int dev, void **p, size_t bytes) {
+    {
+        std::lock_guard<std::mutex> lock(g_cache_mu);
+        for (size_t i = g_cache.size(); i-- > 0;) {
+            if (g_cache[i].dev == dev && g_cache[i].bytes == bytes) {
+                *p = g_cache[i].p;
+                g_cache.erase(g_cache.begin() + (long)i);
+                g_cached_bytes[dev] -= bytes;
+                g_block_bytes[*p] = bytes;
+                return hipSuccess;
+            }
+        }
+    }
+    hipError_t e = hipMalloc(p, bytes);
+    if (e == hipSuccess) {
+        std::lock_guard<std::mutex> lock(g_cache_mu);
+        g_block_bytes[*p] = bytes;
+    }
+    return e;
+}
+
+// release of a block of the current device dev whose work has finished
+static void dev_free(int dev, void *p) {
+    if (!p) return;
+    std::lock_guard<std::mutex> lock(g_cache_mu);
+    auto it = g_block_bytes.find(p);
+    if (it == g_block_bytes.end()) {
+        (void)hipFree(p);
+        return;
+    }
+    const size_t b = it->second;
+    g_block_bytes.erase(it);
+    if (b > cache_cap()) {
+        (void)hipFree(p);
+        return;
+    }
+    g_cache.push_back(CachedBlock{dev, b, p});
+    g_cached_bytes[dev] += b;
+    cache_shrink(dev, cache_cap());
+}
+
+extern "C" int cns_release_cache(int device) {
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    std::lock_guard<std::mutex> lock(g_cache_mu);
+    std::vector<int> devs;
+    for (const auto &kv : g_cached_bytes)
+        if ((device < 0 || kv.first == device) && kv.second > 0) devs.push_back(kv.first);
+    for (int d : devs) {
+        if (hipSetDevice(d) != hipSuccess) continue;
+        cache_shrink(d, 0);
+    }
+    (void)hipSetDevice(cur);
+    return CNS_OK;
+}
+
 static int dalloc(cns_solver *s, double **p, size_t count) {
-    HIPCHK(s, hipMalloc((void **)p, count * sizeof(double)));
+    HIPCHK(s, dev_alloc(s->dev, (void **)p, count * sizeof(double)));
     s->dev_bytes += count * sizeof(double);
     return CNS_OK;
 }
 
 static int balloc(cns_solver *s, void **p, size_t bytes) {
-    HIPCHK(s, hipMalloc(p, bytes));
+    HIPCHK(s, dev_alloc(s->dev, p, bytes));
     s->dev_bytes += bytes;
     return CNS_OK;
 }
@@ -488,9 +590,8 @@ extern "C" int cns_destroy(cns_solver *s) {
     if (!s) return CNS_OK;
     if (s->dev >= 0) (void)hipSetDevice(s->dev);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
-    auto fr = [](double *p) {
-        if (p) (void)hipFree(p);
-    };
+    const int dev = s->dev;
+    auto fr = [dev](double *p) { dev_free(dev, p); };
     fr(s->d_xp);
     fr(s->d_x);
     fr(s->d_y);
@@ -500,12 +601,10 @@ extern "C" int cns_destroy(cns_solver *s) {
     fr(s->d_spx);
     fr(s->d_spz);
     fr(s->d_ref);
-    if (s->d_sc) (void)hipFree(s->d_sc);
-    if (s->d_ctl) (void)hipFree(s->d_ctl);
+    dev_free(dev, s->d_sc);
+    dev_free(dev, s->d_ctl);
     fr(s->d_part);
-    auto fv = [](void *p) {
-        if (p) (void)hipFree(p);
-    };
+    auto fv = [dev](void *p) { dev_free(dev, p); };
     fv(s->d_dil);
     for (int a = 0; a < 4; a++) {
         fv(s->sets[a]);
@@ -591,11 +690,11 @@ extern "C" int cns_create(const cns_config *cfg, cns_solver **out) {
     }
     for (int a = 0; a < s->nrhs; a++)
         if ((rc = balloc(s, &s->rbuf[a], 5 * (size_t)n * s->esz))) return bail(rc);
-    if (hipMalloc((void **)&s->d_ctl, sizeof(StepCtl)) != hipSuccess) {
+    if (dev_alloc(s->dev, (void **)&s->d_ctl, sizeof(StepCtl)) != hipSuccess) {
         s->err = "hipMalloc(step control) failed";
         return bail(CNS_ERR_NOMEM);
     }
-    if (hipMalloc((void **)&s->d_sc, sizeof(Scalars)) != hipSuccess) {
+    if (dev_alloc(s->dev, (void **)&s->d_sc, sizeof(Scalars)) != hipSuccess) {
         s->err = "hipMalloc(scalars) failed";
         return bail(CNS_ERR_NOMEM);
     }
@@ -1253,13 +1352,15 @@ static int queue_steps(cns_solver *s, int nsteps, bool want_par, bool want_time)
 
 static int ensure_par(cns_solver *s, int nsteps) {
     if (nsteps <= s->par_cap) return CNS_OK;
-    if (s->d_par1) (void)hipFree(s->d_par1);
-    if (s->d_par2) (void)hipFree(s->d_par2);
-    if (s->d_time) (void)hipFree(s->d_time);
+    // the arrays may still be in use by queued steps
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    dev_free(s->dev, s->d_par1);
+    dev_free(s->dev, s->d_par2);
+    dev_free(s->dev, s->d_time);
     s->d_par1 = s->d_par2 = s->d_time = nullptr;
-    HIPCHK(s, hipMalloc((void **)&s->d_par1, nsteps * sizeof(double)));
-    HIPCHK(s, hipMalloc((void **)&s->d_par2, nsteps * sizeof(double)));
-    HIPCHK(s, hipMalloc((void **)&s->d_time, nsteps * sizeof(double)));
+    HIPCHK(s, dev_alloc(s->dev, (void **)&s->d_par1, nsteps * sizeof(double)));
+    HIPCHK(s, dev_alloc(s->dev, (void **)&s->d_par2, nsteps * sizeof(double)));
+    HIPCHK(s, dev_alloc(s->dev, (void **)&s->d_time, nsteps * sizeof(double)));
     s->par_cap = nsteps;
     return CNS_OK;
 }
@@ -1711,7 +1812,8 @@ extern "C" int cns_step_stats(cns_solver *s, int nsteps, int every, double *stat
     if (nsamp == 0) return CNS_OK;
     if (4 * nsamp > s->stats_cap) {
         if (s->d_stats) {
-            (void)hipFree(s->d_stats);
+            HIPCHK(s, hipStreamSynchronize(s->stream));
+            dev_free(s->dev, s->d_stats);
             s->dev_bytes -= (uint64_t)s->stats_cap * sizeof(double);
             s->d_stats = nullptr;
             s->stats_cap = 0;
@@ -1970,7 +2072,7 @@ static int upload_jobs(cns_batch *b, BatchGroup &g, int L, bool scratch) {
     }
     g.nwork = first_total;
     g.job_size = sizeof(Job<R>);
-    if (!g.d_jobs) BCHK(b, hipMalloc(&g.d_jobs, h.size() * sizeof(Job<R>)));
+    if (!g.d_jobs) BCHK(b, dev_alloc(b->dev, &g.d_jobs, h.size() * sizeof(Job<R>)));
     BCHK(b, hipMemcpyAsync(g.d_jobs, h.data(), h.size() * sizeof(Job<R>), hipMemcpyHostToDevice, b->stream));
     BCHK(b, hipStreamSynchronize(b->stream));
     return CNS_OK;
@@ -2384,8 +2486,7 @@ extern "C" int cns_batch_destroy(cns_batch *b) {
     if (b->pending) batch_after(b, b->pending);
     b->pending = 0;
     drop_graph(b);
-    for (BatchGroup &g : b->groups)
-        if (g.d_jobs) (void)hipFree(g.d_jobs);
+    for (BatchGroup &g : b->groups) dev_free(b->dev, g.d_jobs);
     for (cns_solver *m : b->members) m->in_batch = false;
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;

@@ -7,8 +7,9 @@ resident until their last step.  Every tick advances all admitted requests
 together with one batch (cns_batch_*: one launch per RK stage per kernel
 group, one replayed graph per step), by the smallest remaining step count or
 the tick length, whichever is less; finished requests leave (a worker
-thread reads their results while the next ticks run, and their device
-memory is freed between ticks), waiting ones are admitted, and the batch is
+thread reads their results while the next ticks run, and their solvers are
+closed between ticks; the library keeps their device blocks cached for the
+next requests of the same sizes), waiting ones are admitted, and the batch is
 rebuilt for the new membership.  Each request's
 result equals its lone cns_step result to rounding (the batch keeps every
 member's step count, so the CFL checks fall on the same steps).
@@ -34,7 +35,7 @@ from concurrent.futures import ThreadPoolExecutor
 import numpy as np
 
 from .fields import load_checkpoint, save_checkpoint
-from .solver import Batch, Solver
+from .solver import Batch, Solver, release_cache
 from .workloads import init_on_device, points, serving
 
 _SETS = {0: (2, 2, 2), 1: (3, 1, 2), 2: (4, 2, 1)}  # scheme -> (state sets, compensated sets, rhs sets)
@@ -105,8 +106,8 @@ class Server:
 
         # Finished requests: their results are read (cns_get_state on their
         # own streams) by a worker thread while the next ticks run; their
-        # solvers are closed on this thread between ticks, when the GPU is
-        # idle anyway (freeing device memory waits for the whole device).
+        # solvers are closed on this thread between ticks (the library keeps
+        # their device blocks for the next solvers of the same sizes).
         reader = ThreadPoolExecutor(max_workers=2)
         reading = []  # (future, entry)
 
@@ -193,6 +194,7 @@ class Server:
         if batch is not None:
             batch.close()
         wall = clock()
+        release_cache()  # closed solvers' device blocks stay cached for reuse until here
         out = summarize(done, wall, busy, ticks)
         out.update({k: round(v, 3) for k, v in cost.items()})
         return done, out

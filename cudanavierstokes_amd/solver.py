@@ -300,6 +300,19 @@ class Solver:
         self._chk(self.L.cns_set_control(self.h, C.byref(c)), "cns_set_control")
 
 
+def mem_info(device=0):
+    """(free, total) bytes of device memory (checkGpuMem's query)"""
+    f, t = C.c_uint64(), C.c_uint64()
+    check(_lib.load().cns_mem_info(int(device), C.byref(f), C.byref(t)), None, "cns_mem_info")
+    return f.value, t.value
+
+
+def release_cache(device=-1):
+    """Free the device blocks that closed solvers and batches left cached for
+    reuse (of one device, or of all for device < 0)."""
+    check(_lib.load().cns_release_cache(int(device)), None, "cns_release_cache")
+
+
 class Batch:
     """Many solvers advanced together (cns_batch_*): one launch per RK stage
     per kernel-template group, one captured graph per step.  A member's

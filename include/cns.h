@@ -260,6 +260,14 @@ int cns_device_bytes(const cns_solver *s, uint64_t *bytes);
  * src/cuda_utils.cu:780-813) */
 int cns_mem_info(int device, uint64_t *free_bytes, uint64_t *total_bytes);
 
+/* No counterpart in the reference (one solver per process there).  Device
+ * blocks that cns_destroy and cns_batch_destroy release stay cached per
+ * device and size for the next solver or batch of the same sizes (freeing
+ * device memory waits for the whole device); at most CNS_CACHE_BYTES bytes
+ * (environment, default 16 GiB) per device.  This frees the cached blocks of
+ * one device, or of every device for device < 0. */
+int cns_release_cache(int device);
+
 /* ---- Batched advance of many solvers (ensembles, the serving loop) ----
  * No counterpart in the reference, which runs one flow per process.  A batch
  * advances its members together: members that share a kernel template
