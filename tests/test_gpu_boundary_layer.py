@@ -133,3 +133,32 @@ def test_boundary_layer_not_batched(prof):
             cns.Batch([a])
     finally:
         a.close()
+
+
+@pytest.mark.timeout(300)
+def test_default_boundary_layer_full_size_runs_stably():
+    """The reference's default case at its own size (240 x 64 x 2048) from
+    the Blasius inflow state: 100 steps stay finite, the density within a few
+    percent of the free stream, the mass within 1e-5 of the start, the
+    wall-normal velocity next to the wall at the blowing's scale (its amplitude
+    is 2e-4 times a profile below 1), and the blowing's spanwise pattern only
+    inside its band."""
+    cfg = cns.boundary_layer()
+    flush_print(f"[case] boundary layer full size {cfg.mx}x{cfg.my}x{cfg.mz}")
+    s = cns.Solver(cfg)
+    try:
+        inflow.calculate_sponge(s, set_state=True)
+        m0 = float(np.sum(s.get_state()[0]))
+        s.step(100, outputs=False)
+        r, u, v, w, e = s.get_state()
+        for a in (r, u, v, w, e):
+            assert np.isfinite(a).all()
+        assert 0.9 < r.min() and r.max() < 1.1
+        assert abs(float(np.sum(r)) / m0 - 1.0) < 1e-5
+        assert np.abs(u[:, :, 0]).max() < 2e-4
+        # the inflow state is uniform along y; only the blowing (through its
+        # spanwise cosine) makes the first node's u vary with y, in its band
+        assert np.std(u[100:121, :, 0], axis=1).max() > 1e-9
+        assert np.std(u[:60, :, 0], axis=1).max() < 1e-14
+    finally:
+        s.close()
