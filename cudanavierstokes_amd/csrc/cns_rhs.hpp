@@ -20,6 +20,8 @@
 #pragma once
 #include "cns_device.hpp"
 
+#include <type_traits>
+
 namespace cns {
 
 // Source of one staged point: the in-plane byte offset of the point whose
@@ -370,6 +372,40 @@ __device__ __forceinline__ void block_max_atomic(double v, unsigned long long *o
 // Bits of the stage-update flags.
 enum { F_Q = 1, F_RA = 2, F_RB = 4, F_ACC = 8, F_R = 16, F_SPONGE = 32, F_VISC = 64, F_COMP = 128, F_QLO = 256 };
 
+// The role-split kernel's plane march is compiled once per common stage
+// kind with its flags as constants, plus once with run-time flags for the
+// rest.  Constant flags fold the update's branches away, so the update of
+// one plane and the stencils of the next form one basic block the compiler
+// can interleave (7.5% on the order-8 FP64 kernel, 1% on the order-6
+// channel).  F_VISC stays a run-time bit (a select, not a branch) and is not
+// part of the key.
+template <int... FS>
+struct FlagList {};
+template <class F>
+__device__ __forceinline__ void with_flags(int, F &f, FlagList<>) {
+    f(std::integral_constant<int, -1>{});
+}
+template <class F, int F0, int... FR>
+__device__ __forceinline__ void with_flags(int key, F &f, FlagList<F0, FR...>) {
+    if (key == F0)
+        f(std::integral_constant<int, F0>{});
+    else
+        with_flags(key, f, FlagList<FR...>{});
+}
+// Stage kinds per storage type: RK4 stages 1-3 and 4; low-storage RK3
+// stages 1, 2 and 3 (Kutta's first two stages have the flags of the first
+// two); with K3 the last Kutta stage.  FP32 storage adds the compensated
+// sums (F_COMP) and the low part of the step result (F_QLO); a sponge
+// (always on in the boundary layer) adds F_SPONGE.
+template <typename R, bool K3, bool SPONGE>
+struct CommonFlags {
+    static constexpr int C = Off<R>::on ? F_COMP : 0, L = Off<R>::on ? F_QLO : 0, SP = SPONGE ? F_SPONGE : 0;
+    using type = std::conditional_t<
+        K3, FlagList<F_Q | F_RA | F_RB | C | L | SP>,
+        FlagList<F_Q | F_ACC | C | SP, F_Q | C | L | SP, F_Q | F_R | C | L | SP, F_Q | F_RA | F_R | C | L | SP,
+                 F_Q | F_RA | C | L | SP>>;
+};
+
 __device__ __forceinline__ long long uniform64(long long v) {
     const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
     const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v >> 32));
@@ -442,10 +478,11 @@ __device__ __forceinline__ void stage_update(const Params &P, const StagePtrs<R>
             for (int q = 0; q < 5; q++) rhs[q] += sg * (R(P.ref[i2 + q * rs]) - ld_at(qc + q * fs, coff));
         }
     }
-    if (fl & F_VISC) {
+    {
+        // a select rather than a branch, so the update stays one basic block
         const double v1 = mu * inv_dx2, v2 = mu * P.d_d2y, v3 = mu * P.d_d2z;
         const double vm = v1 >= (v2 >= v3 ? v2 : v3) ? v1 : (v2 >= v3 ? v2 : v3);
-        vmax = vmax >= vm ? vmax : vm;
+        vmax = ((fl & F_VISC) && !(vmax >= vm)) ? vm : vmax;
     }
     if constexpr (Off<R>::on) {
         // FP64 update of the compensated pair, rounded to hi + lo
@@ -646,6 +683,8 @@ __device__ __forceinline__ void rhs_tile(const Params &P, const StagePtrs<R> &A,
     __syncthreads();  // update parameters in LDS
     const int fl = __builtin_amdgcn_readfirstlane(s_flags);
 
+    // run-time flags (fl): compiled with constant flags, as the role-split
+    // form is, this kernel spills and runs at half speed
     for (int k = z0; k < z1; ++k) {
         const int b = (k - z0) & 1;
         const long poff = (long)k * mxy;
@@ -993,95 +1032,106 @@ __device__ __forceinline__ void rhs_tile_split(const Params &P, const StagePtrs<
         for (int q = 0; q < 5; q++) rx[q] = fy[q] = R(0);
         // the update of plane kk (parity bk) from its x part, its y part
         // (fluxes here, stresses in LDS) and its z part in LDS
-        auto finish = [&](int kk, int bk) {
-            R rhs[5];
-            if constexpr (YS) {
-                rhs[0] = rx[0] + fy[0];
-#pragma unroll
-                for (int q = 1; q < 5; q++) rhs[q] = rx[q] + (fy[q] + xyz[bk][q - 1][tid]);
-            } else {
-#pragma unroll
-                for (int q = 0; q < 5; q++) rhs[q] = rx[q] + fy[q];
-            }
-            rhs[0] = rhs[0] + xyz[bk][4][tid];
-            rhs[1] = rhs[1] + xyz[bk][5][tid];
-            rhs[2] = rhs[2] + xyz[bk][6][tid];
-            rhs[3] = rhs[3] + (xyz[bk][7][tid] + dpdz);
-            rhs[4] = rhs[4] + (xyz[bk][8][tid] + dpdz * wc);
-            stage_update<R, K3>(P, A, arr, fl, s_w, (long)kk * mxy, fs, coff, i, kk, dtd, rhs, ops, muc, inv_dx2,
-                                vmax);
-        };
-        for (int k = z0; k < z1; ++k) {
-            const int b = (k - z0) & 1;
-            {
-                R t[NQ];
-                if constexpr (BL) {
-                    // the source point's primitives; at the bottom the
-                    // adiabatic reflection (the top is formed below)
-                    to_prim<R, false, UNITV>(P, hr, 0, t);
-                    if (hside < 0) {
-                        double ub = 0.0;
-                        const bool blow = P.perturbed && bl_blow(k, bs1, bs2, bcos, ub);
-                        bl_bottom_ghost<R>(P, blow, R(ub), t[QR], t[QU], t[QV], t[QW], t[QT], t[QH]);
-                    }
-                } else {
-                    to_prim<R, WALLX, UNITV>(P, hr, hside, t);
-                }
-                if (hact) {
-#pragma unroll
-                    for (int q = 0; q < NL; q++) sq[b][q][hslot] = t[q];
-                }
-                load_raw(A, (long)wrap1(k + 1, P.mz) * mxy, hoff, hr);
-            }
-            __syncthreads();  // plane k staged; du/dz, dv/dz of plane k; plane k-1's y stresses and z part
-            if constexpr (BL) {
-                if (hact && hside > 0) {
-                    // top x ghost (topBCxExt on u, v, w, p, T, dil; rho, h,
-                    // mu from the ghost p and T: src/boundary_condition_x.h:60-74,
-                    // src/boundary.h:121-148)
-                    const int d = (x0 + hlx - H) - (P.mx - 1);
-                    const int ls = hly * LX + (P.mx - 1 - x0 + H), is = ls - d;
-                    auto ext = [&](int q) { return R(2) * sq[b][q][ls] - sq[b][q][is]; };
-                    const R Rg = R(P.Rgas);
-                    const R ug = ext(QU), vg = ext(QV), wg = ext(QW), tg = ext(QT), dg = ext(QD);
-                    const R pg = R(2) * (sq[b][QR][ls] * Rg * sq[b][QT][ls]) - sq[b][QR][is] * Rg * sq[b][QT][is];
-                    sq[b][QU][hslot] = ug;
-                    sq[b][QV][hslot] = vg;
-                    sq[b][QW][hslot] = wg;
-                    sq[b][QT][hslot] = tg;
-                    sq[b][QD][hslot] = dg;
-                    sq[b][QH][hslot] = tg * R(P.cpR) + R(0.5) * (ug * ug + vg * vg + wg * wg);
-                    sq[b][QR][hslot] = pg / (Rg * tg);
-                    if constexpr (!UNITV) sq[b][QM % NL][hslot] = R(visc_pow((double)tg, P.viscexp)) * R(P.invRe);
-                }
-                __syncthreads();  // x-top ghosts of plane k
-            }
-            if (act) {
-                if (k > z0) finish(k - 1, b ^ 1);
-                const auto FX = FXb(b);
-                const auto FY = FYb(b);
-                R d1x[3], d2x[3];
-                vel_stencils<R, SV, 0, NONUNI>(P, FX, vs, xpi, d1x, d2x);
+        auto march = [&](auto FLC) {
+            constexpr int FC = decltype(FLC)::value;
+            const int flx = FC >= 0 ? FC | (fl & F_VISC) : fl;  // constant but for F_VISC
+            auto finish = [&](int kk, int bk) {
+                R rhs[5];
                 if constexpr (YS) {
-                    const R g[9] = {d1x[0], d1x[1], d1x[2], d1_of<R, SV, 1, false>(P, FY, QU, xpi), R(0), R(0),
-                                    xgz[b][0][tid], xgz[b][1][tid], R(0)};
-                    dir_terms<R, S, SV, 0, NONUNI, UNITV>(P, FX, vs, xpi, g, d2x, rx);
-                    dir_flux<R, S, 1>(P, FY, R(P.d_dy), fy);
+                    rhs[0] = rx[0] + fy[0];
+#pragma unroll
+                    for (int q = 1; q < 5; q++) rhs[q] = rx[q] + (fy[q] + xyz[bk][q - 1][tid]);
                 } else {
-                    R d1y[3], d2y[3];
-                    vel_stencils<R, SV, 1, false>(P, FY, vs, xpi, d1y, d2y);
-                    const R g[9] = {d1x[0], d1x[1], d1x[2], d1y[0], d1y[1], d1y[2],
-                                    xgz[b][0][tid], xgz[b][1][tid], R(0)};
-                    dir_terms<R, S, SV, 0, NONUNI, UNITV>(P, FX, vs, xpi, g, d2x, rx);
-                    dir_terms<R, S, SV, 1, false, UNITV>(P, FY, vs, xpi, g, d2y, fy);
+#pragma unroll
+                    for (int q = 0; q < 5; q++) rhs[q] = rx[q] + fy[q];
                 }
-                wc = FX(QW, 0);
-                muc = (double)FX(QM, 0);
-                load_ops<R, K3>(arr, fl, (long)k * mxy, fs, coff, ops);
-            }
-        }
-        __syncthreads();  // y stresses and z part of the last plane
-        if (act && z1 > z0) finish(z1 - 1, (z1 - 1 - z0) & 1);
+                rhs[0] = rhs[0] + xyz[bk][4][tid];
+                rhs[1] = rhs[1] + xyz[bk][5][tid];
+                rhs[2] = rhs[2] + xyz[bk][6][tid];
+                rhs[3] = rhs[3] + (xyz[bk][7][tid] + dpdz);
+                rhs[4] = rhs[4] + (xyz[bk][8][tid] + dpdz * wc);
+                stage_update<R, K3>(P, A, arr, flx, s_w, (long)kk * mxy, fs, coff, i, kk, dtd, rhs, ops, muc, inv_dx2,
+                                    vmax);
+            };
+            // one plane; FIN: the update of the previous plane runs in it (all
+            // but the first plane of the chunk, which is peeled so the loop body
+            // has no branch between the update and the stencils)
+            auto body = [&](int k, auto FIN) {
+                constexpr bool fin = decltype(FIN)::value;
+                const int b = (k - z0) & 1;
+                {
+                    R t[NQ];
+                    if constexpr (BL) {
+                        // the source point's primitives; at the bottom the
+                        // adiabatic reflection (the top is formed below)
+                        to_prim<R, false, UNITV>(P, hr, 0, t);
+                        if (hside < 0) {
+                            double ub = 0.0;
+                            const bool blow = P.perturbed && bl_blow(k, bs1, bs2, bcos, ub);
+                            bl_bottom_ghost<R>(P, blow, R(ub), t[QR], t[QU], t[QV], t[QW], t[QT], t[QH]);
+                        }
+                    } else {
+                        to_prim<R, WALLX, UNITV>(P, hr, hside, t);
+                    }
+                    if (hact) {
+#pragma unroll
+                        for (int q = 0; q < NL; q++) sq[b][q][hslot] = t[q];
+                    }
+                    load_raw(A, (long)wrap1(k + 1, P.mz) * mxy, hoff, hr);
+                }
+                __syncthreads();  // plane k staged; du/dz, dv/dz of plane k; plane k-1's y stresses and z part
+                if constexpr (BL) {
+                    if (hact && hside > 0) {
+                        // top x ghost (topBCxExt on u, v, w, p, T, dil; rho, h,
+                        // mu from the ghost p and T: src/boundary_condition_x.h:60-74,
+                        // src/boundary.h:121-148)
+                        const int d = (x0 + hlx - H) - (P.mx - 1);
+                        const int ls = hly * LX + (P.mx - 1 - x0 + H), is = ls - d;
+                        auto ext = [&](int q) { return R(2) * sq[b][q][ls] - sq[b][q][is]; };
+                        const R Rg = R(P.Rgas);
+                        const R ug = ext(QU), vg = ext(QV), wg = ext(QW), tg = ext(QT), dg = ext(QD);
+                        const R pg = R(2) * (sq[b][QR][ls] * Rg * sq[b][QT][ls]) - sq[b][QR][is] * Rg * sq[b][QT][is];
+                        sq[b][QU][hslot] = ug;
+                        sq[b][QV][hslot] = vg;
+                        sq[b][QW][hslot] = wg;
+                        sq[b][QT][hslot] = tg;
+                        sq[b][QD][hslot] = dg;
+                        sq[b][QH][hslot] = tg * R(P.cpR) + R(0.5) * (ug * ug + vg * vg + wg * wg);
+                        sq[b][QR][hslot] = pg / (Rg * tg);
+                        if constexpr (!UNITV) sq[b][QM % NL][hslot] = R(visc_pow((double)tg, P.viscexp)) * R(P.invRe);
+                    }
+                    __syncthreads();  // x-top ghosts of plane k
+                }
+                if (act) {
+                    if constexpr (fin) finish(k - 1, b ^ 1);
+                    const auto FX = FXb(b);
+                    const auto FY = FYb(b);
+                    R d1x[3], d2x[3];
+                    vel_stencils<R, SV, 0, NONUNI>(P, FX, vs, xpi, d1x, d2x);
+                    if constexpr (YS) {
+                        const R g[9] = {d1x[0], d1x[1], d1x[2], d1_of<R, SV, 1, false>(P, FY, QU, xpi), R(0), R(0),
+                                        xgz[b][0][tid], xgz[b][1][tid], R(0)};
+                        dir_terms<R, S, SV, 0, NONUNI, UNITV>(P, FX, vs, xpi, g, d2x, rx);
+                        dir_flux<R, S, 1>(P, FY, R(P.d_dy), fy);
+                    } else {
+                        R d1y[3], d2y[3];
+                        vel_stencils<R, SV, 1, false>(P, FY, vs, xpi, d1y, d2y);
+                        const R g[9] = {d1x[0], d1x[1], d1x[2], d1y[0], d1y[1], d1y[2],
+                                        xgz[b][0][tid], xgz[b][1][tid], R(0)};
+                        dir_terms<R, S, SV, 0, NONUNI, UNITV>(P, FX, vs, xpi, g, d2x, rx);
+                        dir_terms<R, S, SV, 1, false, UNITV>(P, FY, vs, xpi, g, d2y, fy);
+                    }
+                    wc = FX(QW, 0);
+                    muc = (double)FX(QM, 0);
+                    load_ops<R, K3>(arr, flx, (long)k * mxy, fs, coff, ops);
+                }
+            };
+            if (z0 < z1) body(z0, std::false_type{});
+            for (int k = z0 + 1; k < z1; ++k) body(k, std::true_type{});
+            __syncthreads();  // y stresses and z part of the last plane
+            if (act && z1 > z0) finish(z1 - 1, (z1 - 1 - z0) & 1);
+        };
+        with_flags(fl & ~F_VISC, march, typename CommonFlags<R, K3, BL>::type{});
     }
     if (P.want_visc_max) block_max_atomic(vmax, P.visc_max_out, red);
 }
