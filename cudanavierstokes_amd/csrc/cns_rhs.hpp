@@ -763,16 +763,17 @@ __device__ __forceinline__ void rhs_tile(const Params &P, const StagePtrs<R> &A,
 }
 
 // Role-split form of rhs_tile for 2 TX TY lanes, one barrier per plane.  The
-// waves of the first half (plane role) stage the x/y halo of each plane,
-// evaluate the x part (and the y part, or its convective fluxes) from LDS
-// and apply the stage update; the waves of the second half (column role)
-// keep the z register window of the tile's columns, stage the interior of
-// each plane and evaluate the z part (and the y stresses).  Each role holds
-// about half the registers of rhs_tile, so two waves share a SIMD and hide
-// each other's latency.  Per plane k (b = parity of k):
-//   column: push plane k+S into the window, stage plane k's interior,
-//           du/dz and dv/dz of plane k -> LDS;
-//   plane:  stage the halo point of plane k;
+// waves of the first half (plane role) evaluate the x part (and the y part,
+// or its convective fluxes) from LDS and apply the stage update; the waves
+// of the second half (column role) keep the z register window of the tile's
+// columns, stage the interior of each plane and evaluate the z part (and the
+// y stresses).  The x/y halo of each plane is staged by the column role with
+// mu = T/Re and by the plane role otherwise (CH).  Each role holds about
+// half the registers of rhs_tile, so two waves share a SIMD and hide each
+// other's latency.  Per plane k (b = parity of k):
+//   column: push plane k+S into the window, stage plane k's interior (and
+//           its halo point), du/dz and dv/dz of plane k -> LDS;
+//   plane:  (stage the halo point of plane k);
 //   -- barrier --
 //   plane:  update of plane k-1 (its z part, and y stresses, came through
 //           LDS), x part and y part or y fluxes of plane k;
@@ -804,6 +805,12 @@ __device__ __forceinline__ void rhs_tile_split(const Params &P, const StagePtrs<
     // takes its stresses (the plane role keeps the fluxes), which balances
     // the roles; with mu = T/Re the plane role evaluates all of it
     constexpr bool YS = !UNITV;
+    // who stages the x/y halo of each plane: the column role when it has no
+    // y stresses to evaluate (it waits at the plane barrier about half of
+    // each plane; 4% on the order-8 mu = T/Re kernel), else the plane role
+    // (10% slower on the order-6 channel with the halo in the column role);
+    // the boundary layer's halo ghosts stay in the plane role
+    constexpr bool CH = !YS && !BL;
     static_assert(NT % 64 == 0, "each role is made of whole waves");
     static_assert(!BL || (WALLX && sizeof(R) == 8), "boundary layer: FP64, non-periodic x");
     constexpr unsigned ESZ = sizeof(R);
@@ -884,7 +891,27 @@ __device__ __forceinline__ void rhs_tile_split(const Params &P, const StagePtrs<
         };
     };
 
+    // this lane's halo point: x-halo columns first, then y-halo rows (lanes
+    // past the ring write an unread corner slot of the plane)
+    const bool hact = tid < NH;
+    int hlx, hly;
+    if (tid < 2 * H * TY) {
+        hly = H + tid / (2 * H);
+        const int c2 = tid % (2 * H);
+        hlx = c2 < H ? c2 : TX + c2;
+    } else {
+        const int s2 = hact ? tid - 2 * H * TY : 0;
+        hlx = H + s2 % TX;
+        const int rr = s2 / TX;
+        hly = rr < H ? rr : TY + rr;
+    }
+    int hside;
+    const unsigned hoff = src_offset<WALLX>(P, x0 + hlx - H, y0 + hly - H, ESZ, hside);
+    const int hslot = hly * LX + hlx;
+    Raw<R> hr;  // prefetched halo point (plane k), in the role that stages it
+
     if (colrole) {
+        if constexpr (CH) load_raw(A, (long)z0 * mxy, hoff, hr);
         constexpr int NW = BL ? NL + 1 : NL;  // window quantities (BL: p last)
         R win[NW][W];
 #pragma unroll
@@ -939,6 +966,14 @@ __device__ __forceinline__ void rhs_tile_split(const Params &P, const StagePtrs<
 #pragma unroll
                 for (int q = 0; q < NL; q++) sq[b][q][cc] = win[q][H];
                 load_raw(A, (long)wrap1(k + 1 + H, P.mz) * mxy, coff, zr);
+                if constexpr (CH) {
+                    R th[NQ];
+                    to_prim<R, WALLX, UNITV>(P, hr, hside, th);
+                    const int ws = hact ? hslot : 0;
+#pragma unroll
+                    for (int q = 0; q < NL; q++) sq[b][q][ws] = th[q];
+                    load_raw(A, (long)wrap1(k + 1, P.mz) * mxy, hoff, hr);
+                }
             }
             vel_stencils<R, SV, 2, false>(P, FZ, nullptr, 1.0, d1z, d2z);
             xgz[b][0][tid] = d1z[0];
@@ -981,22 +1016,6 @@ __device__ __forceinline__ void rhs_tile_split(const Params &P, const StagePtrs<
         // parts, the update), so its waves win issue ties against the column
         // role's (measured: 2.5% on the order-6 channel, neutral at order 8)
         __builtin_amdgcn_s_setprio(3);
-        // this lane's halo point: x-halo columns first, then y-halo rows
-        const bool hact = tid < NH;
-        int hlx, hly;
-        if (tid < 2 * H * TY) {
-            hly = H + tid / (2 * H);
-            const int c2 = tid % (2 * H);
-            hlx = c2 < H ? c2 : TX + c2;
-        } else {
-            const int s2 = hact ? tid - 2 * H * TY : 0;
-            hlx = H + s2 % TX;
-            const int rr = s2 / TX;
-            hly = rr < H ? rr : TY + rr;
-        }
-        int hside;
-        const unsigned hoff = src_offset<WALLX>(P, x0 + hlx - H, y0 + hly - H, ESZ, hside);
-        const int hslot = hly * LX + hlx;
         // BL blowing: sin(omega1 t), sin(10 t) and cos(y/lambda) of this
         // lane's halo row (y = j / d_dy as src/perturbation.h:50 forms it)
         double bs1 = 0.0, bs2 = 0.0, bcos = 0.0;
@@ -1022,8 +1041,7 @@ __device__ __forceinline__ void rhs_tile_split(const Params &P, const StagePtrs<
         }
         const double dtd = (P.write_q || P.write_acc) ? *P.dt : 0.0;
         const R dpdz = P.forcing ? R(*P.dpdz) : R(0);
-        Raw<R> hr;  // prefetched halo point (plane k)
-        load_raw(A, (long)z0 * mxy, hoff, hr);
+        if constexpr (!CH) load_raw(A, (long)z0 * mxy, hoff, hr);
         // the previous plane: x part, y fluxes, update operands, centre w and mu
         R rx[5], fy[5], wc = R(0);
         double muc = 0.0;
@@ -1059,7 +1077,7 @@ __device__ __forceinline__ void rhs_tile_split(const Params &P, const StagePtrs<
             auto body = [&](int k, auto FIN) {
                 constexpr bool fin = decltype(FIN)::value;
                 const int b = (k - z0) & 1;
-                {
+                if constexpr (!CH) {
                     R t[NQ];
                     if constexpr (BL) {
                         // the source point's primitives; at the bottom the
