@@ -767,8 +767,8 @@ __device__ __forceinline__ void rhs_tile(const Params &P, const StagePtrs<R> &A,
 // or its convective fluxes) from LDS and apply the stage update; the waves
 // of the second half (column role) keep the z register window of the tile's
 // columns, stage the interior of each plane and evaluate the z part (and the
-// y stresses).  The x/y halo of each plane is staged by the column role with
-// mu = T/Re and by the plane role otherwise (CH).  Each role holds about
+// y stresses).  The x/y halo of each plane is staged by the column role,
+// in the boundary layer by the plane role (CH).  Each role holds about
 // half the registers of rhs_tile, so two waves share a SIMD and hide each
 // other's latency.  Per plane k (b = parity of k):
 //   column: push plane k+S into the window, stage plane k's interior (and
@@ -778,9 +778,10 @@ __device__ __forceinline__ void rhs_tile(const Params &P, const StagePtrs<R> &A,
 //   plane:  update of plane k-1 (its z part, and y stresses, came through
 //           LDS), x part and y part or y fluxes of plane k;
 //   column: z part (and y stresses) of plane k -> LDS, window shift.
-// The y stresses go to the column role with a general viscosity law, where
-// that balances the two roles best (measured); with mu = T/Re the plane role
-// evaluates the whole y part.  Each role takes the velocity derivatives it
+// The y stresses go to the column role only in the boundary layer with a
+// general viscosity law (YS), where that balances the two roles best
+// (measured); otherwise the plane role evaluates the whole y part.  Each
+// role takes the velocity derivatives it
 // needs from LDS itself, so nothing else crosses between them.  Exchange
 // buffers and the staged planes are double buffered by plane parity.  Both
 // roles run the same barriers; the role is uniform within a wave.  The sums
@@ -800,17 +801,19 @@ __device__ __forceinline__ void rhs_tile_split(const Params &P, const StagePtrs<
     constexpr int H = S, LX = TX + 2 * H, LY = TY + 2 * H, NT = TX * TY, W = 2 * H + 1;
     constexpr int NL = UNITV ? NQ - 1 : NQ;  // staged quantities (mu rebuilt from T when mu = T/Re)
     constexpr int NH = 2 * H * TY + 2 * H * TX;
-    static_assert(NH <= NT, "one halo point per plane-role lane");
-    // who evaluates the y part: with a general viscosity law the column role
-    // takes its stresses (the plane role keeps the fluxes), which balances
-    // the roles; with mu = T/Re the plane role evaluates all of it
-    constexpr bool YS = !UNITV;
-    // who stages the x/y halo of each plane: the column role when it has no
-    // y stresses to evaluate (it waits at the plane barrier about half of
-    // each plane; 4% on the order-8 mu = T/Re kernel), else the plane role
-    // (10% slower on the order-6 channel with the halo in the column role);
-    // the boundary layer's halo ghosts stay in the plane role
-    constexpr bool CH = !YS && !BL;
+    static_assert(NH <= NT, "one halo point per lane");
+    // who stages the x/y halo of each plane: the column role (it would wait
+    // at the plane barrier about half of each plane otherwise), except in the
+    // boundary layer, whose halo ghosts the plane role forms
+    constexpr bool CH = !BL;
+    // who evaluates the y part: the plane role, except the y stresses with a
+    // general viscosity law in the boundary layer, where the column role has
+    // no halo to stage and takes them (the plane role keeps the fluxes).
+    // Measured: with the halo in the column role the plane role's full y
+    // part is 12% faster on the order-6 channel than the y stresses in the
+    // column role; in the boundary layer the stresses in the column role are
+    // 7% faster
+    constexpr bool YS = BL && !UNITV;
     static_assert(NT % 64 == 0, "each role is made of whole waves");
     static_assert(!BL || (WALLX && sizeof(R) == 8), "boundary layer: FP64, non-periodic x");
     constexpr unsigned ESZ = sizeof(R);
