@@ -214,11 +214,9 @@ __device__ __forceinline__ R d1_of(const Params &P, Acc f, int q, double xpi) {
     return a1;
 }
 
-// FLUX = false leaves the convective fluxes out (zero), for a caller that
-// evaluates them separately with dir_flux.  PQ takes the pressure from the
-// staged quantity QP (p itself) instead of forming rho R T at each point.
-template <typename R, int S, int SV, int DIR, bool NONUNI, bool UNITV, bool FLUX = true, bool PQ = false,
-          class Acc>
+// PQ takes the pressure from the staged quantity QP (p itself) instead of
+// forming rho R T at each point.
+template <typename R, int S, int SV, int DIR, bool NONUNI, bool UNITV, bool PQ = false, class Acc>
 __device__ __forceinline__ DirA<R> dir_terms_a(const Params &P, Acc f, const double *vs, double xpi) {
     const R d = R(DIR == 0 ? P.d_dx : (DIR == 1 ? P.d_dy : P.d_dz));
     const R dd = R(DIR == 0 ? P.d_d2x : (DIR == 1 ? P.d_d2y : P.d_d2z));
@@ -278,8 +276,8 @@ __device__ __forceinline__ DirA<R> dir_terms_a(const Params &P, Acc f, const dou
     d1p = PQ ? d1p * xs : d1p * (Rg * xs);
     const R d1l = d1m * R(P.invPrEc);
 
-    R fl[5] = {R(0), R(0), R(0), R(0), R(0)};
-    if constexpr (FLUX) dir_flux<R, S, DIR>(P, f, xs, fl);
+    R fl[5];
+    dir_flux<R, S, DIR>(P, f, xs, fl);
     const R cr = fl[0], cu = fl[1], cv = fl[2], cw = fl[3], ce = fl[4];
     return DirA<R>{uc, vc, wc, muc, dilc, lamc, d1m, d1t, d1d, d2t, d1p, d1l, cr, cu, cv, cw, ce};
 }
@@ -352,7 +350,7 @@ __device__ __forceinline__ void dir_terms_b(const Params &P, const DirA<R> &a, c
 template <typename R, int S, int SV, int DIR, bool NONUNI, bool UNITV, bool PQ = false, class Acc>
 __device__ __forceinline__ void dir_terms(const Params &P, Acc f, const double *vs, double xpi, const R g[9],
                                          const R d2v[3], R out[5]) {
-    dir_terms_b<R, DIR>(P, dir_terms_a<R, S, SV, DIR, NONUNI, UNITV, true, PQ>(P, f, vs, xpi), g, d2v, out);
+    dir_terms_b<R, DIR>(P, dir_terms_a<R, S, SV, DIR, NONUNI, UNITV, PQ>(P, f, vs, xpi), g, d2v, out);
 }
 
 __device__ __forceinline__ void block_max_atomic(double v, unsigned long long *out, double *red) {
@@ -763,30 +761,28 @@ __device__ __forceinline__ void rhs_tile(const Params &P, const StagePtrs<R> &A,
 }
 
 // Role-split form of rhs_tile for 2 TX TY lanes, one barrier per plane.  The
-// waves of the first half (plane role) evaluate the x part (and the y part,
-// or its convective fluxes) from LDS and apply the stage update; the waves
-// of the second half (column role) keep the z register window of the tile's
-// columns, stage the interior of each plane and evaluate the z part (and the
-// y stresses).  The x/y halo of each plane is staged by the column role,
-// in the boundary layer by the plane role (CH).  Each role holds about
+// waves of the first half (plane role) evaluate the x and y parts from LDS
+// and apply the stage update; the waves of the second half (column role)
+// keep the z register window of the tile's columns, stage each plane (its
+// interior and its x/y halo) and evaluate the z part.  Each role holds about
 // half the registers of rhs_tile, so two waves share a SIMD and hide each
 // other's latency.  Per plane k (b = parity of k):
-//   column: push plane k+S into the window, stage plane k's interior (and
-//           its halo point), du/dz and dv/dz of plane k -> LDS;
-//   plane:  (stage the halo point of plane k);
+//   column: push plane k+S into the window, stage plane k's interior and
+//           one halo point per lane, du/dz and dv/dz of plane k -> LDS;
 //   -- barrier --
-//   plane:  update of plane k-1 (its z part, and y stresses, came through
-//           LDS), x part and y part or y fluxes of plane k;
-//   column: z part (and y stresses) of plane k -> LDS, window shift.
-// The y stresses go to the column role only in the boundary layer with a
-// general viscosity law (YS), where that balances the two roles best
-// (measured); otherwise the plane role evaluates the whole y part.  Each
-// role takes the velocity derivatives it
-// needs from LDS itself, so nothing else crosses between them.  Exchange
-// buffers and the staged planes are double buffered by plane parity.  Both
-// roles run the same barriers; the role is uniform within a wave.  The sums
-// follow rhs_tile: (x part + y part) + z part, as src/cuda_rhs.cu
-// accumulates them.
+//   plane:  update of plane k-1 (its z part came through LDS), x part and
+//           y part of plane k;
+//   column: z part of plane k -> LDS, window shift.
+// The plane role is the critical path: the column role takes all the work
+// that crosses no role boundary (the staging; measured: the plane role
+// waited at the barrier 3% of its time and the column role 52% with the
+// halo in the plane role, and moving it gained 4% at order 8 and 12% on
+// the order-6 channel; moving y stresses or the update instead, which
+// then cross through LDS, measured slower).  Each role takes the velocity
+// derivatives it needs from LDS itself.  Exchange buffers and the staged
+// planes are double buffered by plane parity.  Both roles run the same
+// barriers; the role is uniform within a wave.  The sums follow rhs_tile:
+// (x part + y part) + z part, as src/cuda_rhs.cu accumulates them.
 // BL: boundary-layer ghosts (src/boundary_condition_x.h, _z.h with
 // boundaryLayer).  The bottom x ghosts are staged by their halo lanes
 // (adiabatic reflection, blowing); the top x ghosts extrapolate the tile's
@@ -801,28 +797,16 @@ __device__ __forceinline__ void rhs_tile_split(const Params &P, const StagePtrs<
     constexpr int H = S, LX = TX + 2 * H, LY = TY + 2 * H, NT = TX * TY, W = 2 * H + 1;
     constexpr int NL = UNITV ? NQ - 1 : NQ;  // staged quantities (mu rebuilt from T when mu = T/Re)
     constexpr int NH = 2 * H * TY + 2 * H * TX;
-    static_assert(NH <= NT, "one halo point per lane");
-    // who stages the x/y halo of each plane: the column role (it would wait
-    // at the plane barrier about half of each plane otherwise), except in the
-    // boundary layer, whose halo ghosts the plane role forms
-    constexpr bool CH = !BL;
-    // who evaluates the y part: the plane role, except the y stresses with a
-    // general viscosity law in the boundary layer, where the column role has
-    // no halo to stage and takes them (the plane role keeps the fluxes).
-    // Measured: with the halo in the column role the plane role's full y
-    // part is 12% faster on the order-6 channel than the y stresses in the
-    // column role; in the boundary layer the stresses in the column role are
-    // 7% faster
-    constexpr bool YS = BL && !UNITV;
+    static_assert(NH <= NT, "one halo point per column-role lane");
     static_assert(NT % 64 == 0, "each role is made of whole waves");
     static_assert(!BL || (WALLX && sizeof(R) == 8), "boundary layer: FP64, non-periodic x");
     constexpr unsigned ESZ = sizeof(R);
     __shared__ R sq[2][NL][LY * LX];
     // column -> plane role: du/dz, dv/dz of plane k (used after the barrier
-    // of plane k) and, from plane k, the y stresses (4) and the z part (5)
-    // (used after the barrier of plane k+1)
+    // of plane k) and the z part of plane k (used after the barrier of
+    // plane k+1)
     __shared__ R xgz[2][2][NT];
-    __shared__ R xyz[2][9][NT];
+    __shared__ R xyz[2][5][NT];
     __shared__ double red[2 * NT / 64 + 1];
     __shared__ double s_w[4];
     __shared__ long long s_off[11];
@@ -912,9 +896,61 @@ __device__ __forceinline__ void rhs_tile_split(const Params &P, const StagePtrs<
     const unsigned hoff = src_offset<WALLX>(P, x0 + hlx - H, y0 + hly - H, ESZ, hside);
     const int hslot = hly * LX + hlx;
     Raw<R> hr;  // prefetched halo point (plane k), in the role that stages it
+    // BL blowing: sin(omega1 t), sin(10 t) and cos(y/lambda) of this lane's
+    // halo row (y = j / d_dy as src/perturbation.h:50 forms it)
+    double bs1 = 0.0, bs2 = 0.0, bcos = 0.0;
+    if constexpr (BL) {
+        if (P.perturbed) {
+            const double tn = *P.time;
+            bs1 = sin(P.blow_om1 * tn);
+            bs2 = sin(10.0 * tn);
+            bcos = cos(((double)wrapi(y0 + hly - H, P.my) / P.d_dy) / P.blow_lam);
+        }
+    }
+    // the halo point of plane k into LDS buffer b, then the prefetch of
+    // plane k+1; BL: the source point's primitives, at the bottom the
+    // adiabatic reflection with the blowing (the top is formed by top_ghost)
+    auto stage_halo = [&](int k, int b) {
+        R t[NQ];
+        if constexpr (BL) {
+            to_prim<R, false, UNITV>(P, hr, 0, t);
+            if (hside < 0) {
+                double ub = 0.0;
+                const bool blow = P.perturbed && bl_blow(k, bs1, bs2, bcos, ub);
+                bl_bottom_ghost<R>(P, blow, R(ub), t[QR], t[QU], t[QV], t[QW], t[QT], t[QH]);
+            }
+        } else {
+            to_prim<R, WALLX, UNITV>(P, hr, hside, t);
+        }
+        const int ws = hact ? hslot : 0;
+#pragma unroll
+        for (int q = 0; q < NL; q++) sq[b][q][ws] = t[q];
+        load_raw(A, (long)wrap1(k + 1, P.mz) * mxy, hoff, hr);
+    };
+    // BL: the top x ghost of the halo point from the staged plane (topBCxExt
+    // on u, v, w, p, T, dil; rho, h, mu from the ghost p and T:
+    // src/boundary_condition_x.h:60-74, src/boundary.h:121-148)
+    auto top_ghost = [&](int b) {
+        if (hact && hside > 0) {
+            const int d = (x0 + hlx - H) - (P.mx - 1);
+            const int ls = hly * LX + (P.mx - 1 - x0 + H), is = ls - d;
+            auto ext = [&](int q) { return R(2) * sq[b][q][ls] - sq[b][q][is]; };
+            const R Rg = R(P.Rgas);
+            const R ug = ext(QU), vg = ext(QV), wg = ext(QW), tg = ext(QT), dg = ext(QD);
+            const R pg = R(2) * (sq[b][QR][ls] * Rg * sq[b][QT][ls]) - sq[b][QR][is] * Rg * sq[b][QT][is];
+            sq[b][QU][hslot] = ug;
+            sq[b][QV][hslot] = vg;
+            sq[b][QW][hslot] = wg;
+            sq[b][QT][hslot] = tg;
+            sq[b][QD][hslot] = dg;
+            sq[b][QH][hslot] = tg * R(P.cpR) + R(0.5) * (ug * ug + vg * vg + wg * wg);
+            sq[b][QR][hslot] = pg / (Rg * tg);
+            if constexpr (!UNITV) sq[b][QM % NL][hslot] = R(visc_pow((double)tg, P.viscexp)) * R(P.invRe);
+        }
+    };
 
     if (colrole) {
-        if constexpr (CH) load_raw(A, (long)z0 * mxy, hoff, hr);
+        load_raw(A, (long)z0 * mxy, hoff, hr);
         constexpr int NW = BL ? NL + 1 : NL;  // window quantities (BL: p last)
         R win[NW][W];
 #pragma unroll
@@ -969,67 +1005,37 @@ __device__ __forceinline__ void rhs_tile_split(const Params &P, const StagePtrs<
 #pragma unroll
                 for (int q = 0; q < NL; q++) sq[b][q][cc] = win[q][H];
                 load_raw(A, (long)wrap1(k + 1 + H, P.mz) * mxy, coff, zr);
-                if constexpr (CH) {
-                    R th[NQ];
-                    to_prim<R, WALLX, UNITV>(P, hr, hside, th);
-                    const int ws = hact ? hslot : 0;
-#pragma unroll
-                    for (int q = 0; q < NL; q++) sq[b][q][ws] = th[q];
-                    load_raw(A, (long)wrap1(k + 1, P.mz) * mxy, hoff, hr);
-                }
+                stage_halo(k, b);
             }
             vel_stencils<R, SV, 2, false>(P, FZ, nullptr, 1.0, d1z, d2z);
             xgz[b][0][tid] = d1z[0];
             xgz[b][1][tid] = d1z[1];
             __syncthreads();  // plane k staged; du/dz, dv/dz of plane k
-            if constexpr (BL) __syncthreads();  // x-top ghosts of plane k
+            if constexpr (BL) {
+                top_ghost(b);
+                __syncthreads();  // x-top ghosts of plane k
+            }
             if (act) {
                 const auto FX = FXb(b);
                 const auto FY = FYb(b);
                 R part[5];
-                if constexpr (YS) {
-                    R d1y[3], d2y[3];
-                    vel_stencils<R, SV, 1, false>(P, FY, nullptr, 1.0, d1y, d2y);
-                    const R g[9] = {R(0),   d1_of<R, SV, 0, NONUNI>(P, FX, QV, xpi),
-                                    d1_of<R, SV, 0, NONUNI>(P, FX, QW, xpi),
-                                    d1y[0], d1y[1], d1y[2], d1z[0], d1z[1], d1z[2]};
-                    dir_terms<R, S, SV, 2, false, UNITV, BL>(P, FZ, nullptr, 1.0, g, d2z, part);
+                const R g[9] = {R(0), R(0), d1_of<R, SV, 0, NONUNI>(P, FX, QW, xpi), R(0), R(0),
+                                d1_of<R, SV, 1, false>(P, FY, QW, xpi), d1z[0], d1z[1], d1z[2]};
+                dir_terms<R, S, SV, 2, false, UNITV, BL>(P, FZ, nullptr, 1.0, g, d2z, part);
 #pragma unroll
-                    for (int q = 0; q < 5; q++) xyz[b][4 + q][tid] = part[q];
-                    dir_terms_b<R, 1>(P, dir_terms_a<R, S, SV, 1, false, UNITV, false>(P, FY, nullptr, 1.0), g, d2y,
-                                      part);
-#pragma unroll
-                    for (int q = 1; q < 5; q++) xyz[b][q - 1][tid] = part[q];
-                } else {
-                    const R g[9] = {R(0), R(0), d1_of<R, SV, 0, NONUNI>(P, FX, QW, xpi), R(0), R(0),
-                                    d1_of<R, SV, 1, false>(P, FY, QW, xpi), d1z[0], d1z[1], d1z[2]};
-                    dir_terms<R, S, SV, 2, false, UNITV, BL>(P, FZ, nullptr, 1.0, g, d2z, part);
-#pragma unroll
-                    for (int q = 0; q < 5; q++) xyz[b][4 + q][tid] = part[q];
-                }
+                for (int q = 0; q < 5; q++) xyz[b][q][tid] = part[q];
             }
 #pragma unroll
             for (int q = 0; q < NW; q++)
 #pragma unroll
                 for (int a = 0; a < W - 1; a++) win[q][a] = win[q][a + 1];
         }
-        __syncthreads();  // y stresses and z part of the last plane
+        __syncthreads();  // z part of the last plane
     } else {
-        // the plane role carries the longer chain per plane (halo, x and y
-        // parts, the update), so its waves win issue ties against the column
-        // role's (measured: 2.5% on the order-6 channel, neutral at order 8)
+        // the plane role carries the longer chain per plane (x and y parts,
+        // the update), so its waves win issue ties against the column role's
+        // (measured: 2.5% on the order-6 channel, neutral at order 8)
         __builtin_amdgcn_s_setprio(3);
-        // BL blowing: sin(omega1 t), sin(10 t) and cos(y/lambda) of this
-        // lane's halo row (y = j / d_dy as src/perturbation.h:50 forms it)
-        double bs1 = 0.0, bs2 = 0.0, bcos = 0.0;
-        if constexpr (BL) {
-            if (P.perturbed) {
-                const double tn = *P.time;
-                bs1 = sin(P.blow_om1 * tn);
-                bs2 = sin(10.0 * tn);
-                bcos = cos(((double)wrapi(y0 + hly - H, P.my) / P.d_dy) / P.blow_lam);
-            }
-        }
         double inv_dx2 = 1.0;
         double vs[2 * SV + 1];
 #pragma unroll
@@ -1044,33 +1050,26 @@ __device__ __forceinline__ void rhs_tile_split(const Params &P, const StagePtrs<
         }
         const double dtd = (P.write_q || P.write_acc) ? *P.dt : 0.0;
         const R dpdz = P.forcing ? R(*P.dpdz) : R(0);
-        if constexpr (!CH) load_raw(A, (long)z0 * mxy, hoff, hr);
-        // the previous plane: x part, y fluxes, update operands, centre w and mu
+        // the previous plane: x part, y part, update operands, centre w and mu
         R rx[5], fy[5], wc = R(0);
         double muc = 0.0;
         Ops<R> ops;
 #pragma unroll
         for (int q = 0; q < 5; q++) rx[q] = fy[q] = R(0);
-        // the update of plane kk (parity bk) from its x part, its y part
-        // (fluxes here, stresses in LDS) and its z part in LDS
+        // the update of plane kk (parity bk) from its x and y parts and its z
+        // part in LDS
         auto march = [&](auto FLC) {
             constexpr int FC = decltype(FLC)::value;
             const int flx = FC >= 0 ? FC | (fl & F_VISC) : fl;  // constant but for F_VISC
             auto finish = [&](int kk, int bk) {
                 R rhs[5];
-                if constexpr (YS) {
-                    rhs[0] = rx[0] + fy[0];
 #pragma unroll
-                    for (int q = 1; q < 5; q++) rhs[q] = rx[q] + (fy[q] + xyz[bk][q - 1][tid]);
-                } else {
-#pragma unroll
-                    for (int q = 0; q < 5; q++) rhs[q] = rx[q] + fy[q];
-                }
-                rhs[0] = rhs[0] + xyz[bk][4][tid];
-                rhs[1] = rhs[1] + xyz[bk][5][tid];
-                rhs[2] = rhs[2] + xyz[bk][6][tid];
-                rhs[3] = rhs[3] + (xyz[bk][7][tid] + dpdz);
-                rhs[4] = rhs[4] + (xyz[bk][8][tid] + dpdz * wc);
+                for (int q = 0; q < 5; q++) rhs[q] = rx[q] + fy[q];
+                rhs[0] = rhs[0] + xyz[bk][0][tid];
+                rhs[1] = rhs[1] + xyz[bk][1][tid];
+                rhs[2] = rhs[2] + xyz[bk][2][tid];
+                rhs[3] = rhs[3] + (xyz[bk][3][tid] + dpdz);
+                rhs[4] = rhs[4] + (xyz[bk][4][tid] + dpdz * wc);
                 stage_update<R, K3>(P, A, arr, flx, s_w, (long)kk * mxy, fs, coff, i, kk, dtd, rhs, ops, muc, inv_dx2,
                                     vmax);
             };
@@ -1080,68 +1079,20 @@ __device__ __forceinline__ void rhs_tile_split(const Params &P, const StagePtrs<
             auto body = [&](int k, auto FIN) {
                 constexpr bool fin = decltype(FIN)::value;
                 const int b = (k - z0) & 1;
-                if constexpr (!CH) {
-                    R t[NQ];
-                    if constexpr (BL) {
-                        // the source point's primitives; at the bottom the
-                        // adiabatic reflection (the top is formed below)
-                        to_prim<R, false, UNITV>(P, hr, 0, t);
-                        if (hside < 0) {
-                            double ub = 0.0;
-                            const bool blow = P.perturbed && bl_blow(k, bs1, bs2, bcos, ub);
-                            bl_bottom_ghost<R>(P, blow, R(ub), t[QR], t[QU], t[QV], t[QW], t[QT], t[QH]);
-                        }
-                    } else {
-                        to_prim<R, WALLX, UNITV>(P, hr, hside, t);
-                    }
-                    if (hact) {
-#pragma unroll
-                        for (int q = 0; q < NL; q++) sq[b][q][hslot] = t[q];
-                    }
-                    load_raw(A, (long)wrap1(k + 1, P.mz) * mxy, hoff, hr);
-                }
-                __syncthreads();  // plane k staged; du/dz, dv/dz of plane k; plane k-1's y stresses and z part
-                if constexpr (BL) {
-                    if (hact && hside > 0) {
-                        // top x ghost (topBCxExt on u, v, w, p, T, dil; rho, h,
-                        // mu from the ghost p and T: src/boundary_condition_x.h:60-74,
-                        // src/boundary.h:121-148)
-                        const int d = (x0 + hlx - H) - (P.mx - 1);
-                        const int ls = hly * LX + (P.mx - 1 - x0 + H), is = ls - d;
-                        auto ext = [&](int q) { return R(2) * sq[b][q][ls] - sq[b][q][is]; };
-                        const R Rg = R(P.Rgas);
-                        const R ug = ext(QU), vg = ext(QV), wg = ext(QW), tg = ext(QT), dg = ext(QD);
-                        const R pg = R(2) * (sq[b][QR][ls] * Rg * sq[b][QT][ls]) - sq[b][QR][is] * Rg * sq[b][QT][is];
-                        sq[b][QU][hslot] = ug;
-                        sq[b][QV][hslot] = vg;
-                        sq[b][QW][hslot] = wg;
-                        sq[b][QT][hslot] = tg;
-                        sq[b][QD][hslot] = dg;
-                        sq[b][QH][hslot] = tg * R(P.cpR) + R(0.5) * (ug * ug + vg * vg + wg * wg);
-                        sq[b][QR][hslot] = pg / (Rg * tg);
-                        if constexpr (!UNITV) sq[b][QM % NL][hslot] = R(visc_pow((double)tg, P.viscexp)) * R(P.invRe);
-                    }
-                    __syncthreads();  // x-top ghosts of plane k
-                }
+                __syncthreads();  // plane k staged; du/dz, dv/dz of plane k; plane k-1's z part
+                if constexpr (BL) __syncthreads();  // x-top ghosts of plane k
                 if (act) {
                     if constexpr (fin) finish(k - 1, b ^ 1);
                     const auto FX = FXb(b);
                     const auto FY = FYb(b);
                     R d1x[3], d2x[3];
                     vel_stencils<R, SV, 0, NONUNI>(P, FX, vs, xpi, d1x, d2x);
-                    if constexpr (YS) {
-                        const R g[9] = {d1x[0], d1x[1], d1x[2], d1_of<R, SV, 1, false>(P, FY, QU, xpi), R(0), R(0),
-                                        xgz[b][0][tid], xgz[b][1][tid], R(0)};
-                        dir_terms<R, S, SV, 0, NONUNI, UNITV>(P, FX, vs, xpi, g, d2x, rx);
-                        dir_flux<R, S, 1>(P, FY, R(P.d_dy), fy);
-                    } else {
-                        R d1y[3], d2y[3];
-                        vel_stencils<R, SV, 1, false>(P, FY, vs, xpi, d1y, d2y);
-                        const R g[9] = {d1x[0], d1x[1], d1x[2], d1y[0], d1y[1], d1y[2],
-                                        xgz[b][0][tid], xgz[b][1][tid], R(0)};
-                        dir_terms<R, S, SV, 0, NONUNI, UNITV>(P, FX, vs, xpi, g, d2x, rx);
-                        dir_terms<R, S, SV, 1, false, UNITV>(P, FY, vs, xpi, g, d2y, fy);
-                    }
+                    R d1y[3], d2y[3];
+                    vel_stencils<R, SV, 1, false>(P, FY, vs, xpi, d1y, d2y);
+                    const R g[9] = {d1x[0], d1x[1], d1x[2], d1y[0], d1y[1], d1y[2],
+                                    xgz[b][0][tid], xgz[b][1][tid], R(0)};
+                    dir_terms<R, S, SV, 0, NONUNI, UNITV>(P, FX, vs, xpi, g, d2x, rx);
+                    dir_terms<R, S, SV, 1, false, UNITV>(P, FY, vs, xpi, g, d2y, fy);
                     wc = FX(QW, 0);
                     muc = (double)FX(QM, 0);
                     load_ops<R, K3>(arr, flx, (long)k * mxy, fs, coff, ops);
@@ -1149,7 +1100,7 @@ __device__ __forceinline__ void rhs_tile_split(const Params &P, const StagePtrs<
             };
             if (z0 < z1) body(z0, std::false_type{});
             for (int k = z0 + 1; k < z1; ++k) body(k, std::true_type{});
-            __syncthreads();  // y stresses and z part of the last plane
+            __syncthreads();  // z part of the last plane
             if (act && z1 > z0) finish(z1 - 1, (z1 - 1 - z0) & 1);
         };
         with_flags(fl & ~F_VISC, march, typename CommonFlags<R, K3, BL>::type{});
