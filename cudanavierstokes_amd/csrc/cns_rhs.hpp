@@ -1034,7 +1034,7 @@ __device__ __forceinline__ void rhs_tile_split(const Params &P, const StagePtrs<
     } else {
         // the plane role carries the longer chain per plane (x and y parts,
         // the update), so its waves win issue ties against the column role's
-        // (measured: 2.5% on the order-6 channel, neutral at order 8)
+        // (measured: 2.5-4% on the order-6 channel, neutral at order 8)
         __builtin_amdgcn_s_setprio(3);
         double inv_dx2 = 1.0;
         double vs[2 * SV + 1];
