@@ -687,11 +687,13 @@ __device__ __forceinline__ void rhs_tile(const Params &P, const StagePtrs<R> &A,
         const int b = (k - z0) & 1;
         const long poff = (long)k * mxy;
         // Operands of this plane's update, issued before the window feed and
-        // halo prefetches with FP32 storage, after them (shorter register
-        // lifetimes) with FP64, which measured faster each way.
+        // halo prefetches with FP32 storage at order 8, after the plane
+        // barrier (shorter register lifetimes) otherwise, which measured
+        // faster each way (FP32: 2% at order 8; 4% on the order-6 channel and
+        // 3-6% on the 128^3 order-6 Taylor-Green case the other way).
         Ops<R> ops;
         auto load_update_operands = [&]() { load_ops<R, K3>(arr, fl, poff, fs, coff, ops); };
-        constexpr bool early_ops = sizeof(R) == 4;
+        constexpr bool early_ops = sizeof(R) == 4 && S == 4;
         if constexpr (early_ops) load_update_operands();
         __builtin_amdgcn_sched_barrier(0);
         {
